@@ -1,0 +1,126 @@
+/*
+ * kamino.h -- C ABI of the MI355X-native KaminoGPU time step (libkamino.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (KAM_OK = 0, negative on error).  Nothing here exits the process; the text
+ * of the last error on the calling thread is returned by kam_last_error().
+ * Handles are thread-compatible, not thread-safe.  Caller buffers are host
+ * memory owned by the caller, dense theta-major with phi contiguous:
+ *     u_phi[nTheta][nPhi], u_theta[nTheta-1][nPhi], rho[nTheta][nPhi]
+ * (the reference's KaminoQuantity layout, KaminoQuantity.cu:70-73, without
+ * the pitch).  Reference citations are relative to the KaminoGPU checkout.
+ */
+#ifndef KAMINO_H
+#define KAMINO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    KAM_OK = 0,
+    KAM_EINVAL = -1,   /* bad argument or unsupported shape */
+    KAM_ENOMEM = -2,   /* device or host allocation failed */
+    KAM_EHIP = -3,     /* a HIP runtime call failed */
+    KAM_EAGAIN = -4,   /* engine has no slot / HBM budget for the request now */
+    KAM_ENOTFOUND = -5,/* unknown request id */
+    KAM_ESTATE = -6,   /* call not valid in the object's current state */
+    KAM_ENODEV = -7    /* no usable gfx950 device */
+};
+
+/* ---- step phases (kam_sim_step_phases) ----------------------------------- */
+enum {
+    KAM_PHASE_ADVECT = 1,    /* advection (+ the geometric update if enabled) */
+    KAM_PHASE_DIVFFT = 8,    /* divergence + row FFT into the spectrum buffer */
+    KAM_PHASE_TRI = 16,      /* per-mode tridiagonal solves, in place */
+    KAM_PHASE_IFFT = 32,     /* inverse row FFT + gradient subtraction */
+    KAM_PHASE_PROJECT = 8 | 16 | 32,
+    KAM_PHASE_ALL = 1 | 8 | 16 | 32
+};
+
+/* ---- precision modes ----------------------------------------------------- */
+enum {
+    KAM_PREC_FP32 = 0  /* FP32 fields, advection and FFT; FP64 tridiagonal solves */
+};
+
+/* Shape and parameters of one simulation.  Replaces the KaminoSolver ctor
+ * arguments (KaminoSolver.cuh:99-100) and the __constant__ grid uploads the
+ * caller had to make (KaminoCore.cu:868-872): nPhi = 2 nTheta, h = pi/nTheta
+ * (KaminoCore.cu:849). */
+typedef struct kam_sim_desc {
+    int n_theta;     /* latitude rows, >= 4; nTheta must be 2^a 3^b 5^c */
+    int n_phi;       /* must equal 2 * n_theta */
+    double radius;   /* sphere radius R */
+    double dt;       /* time step (constant, as timeStepGlobal, KaminoCore.cu:871) */
+    int geometric;   /* 1: the reference step (advection, geometric, projection,
+                        KaminoSolver.cu:197-221); 0: advection + projection only */
+    int precision;   /* KAM_PREC_* */
+    int device;      /* HIP device ordinal */
+} kam_sim_desc;
+
+typedef struct kam_sim kam_sim;
+
+/* Text of the last error on this thread ("" if none). */
+const char *kam_last_error(void);
+
+/* Library version string. */
+const char *kam_version(void);
+
+/* Number of gfx950 devices visible (0 if none). */
+int kam_device_count(void);
+
+/* Creates one simulation with its own stream and device buffers (28 bytes per
+ * cell).  Replaces KaminoSolver::KaminoSolver (KaminoSolver.cu:12-67). */
+int kam_sim_create(const kam_sim_desc *desc, kam_sim **out);
+
+/* Frees the simulation.  Replaces KaminoSolver::~KaminoSolver
+ * (KaminoSolver.cu:69-104), without its cudaDeviceReset. */
+int kam_sim_destroy(kam_sim *sim);
+
+/* Uploads the state (host, FP32).  Replaces KaminoQuantity::copyToGPU
+ * (KaminoQuantity.cu:3-12) for u_phi, u_theta and rho.  rho may be NULL. */
+int kam_sim_set_fields(kam_sim *sim, const float *u_phi, const float *u_theta, const float *rho);
+
+/* Downloads the state after all queued steps.  Replaces
+ * KaminoQuantity::copyBackToCPU (KaminoQuantity.cu:14-18).  Any pointer may
+ * be NULL. */
+int kam_sim_get_fields(kam_sim *sim, float *u_phi, float *u_theta, float *rho);
+
+/* Queues nsteps full steps on the simulation's stream.  Replaces
+ * KaminoSolver::stepForward (KaminoSolver.cu:197-221); like the reference's
+ * kernels, every step uses the creation-time dt. */
+int kam_sim_step(kam_sim *sim, int nsteps);
+
+/* Queues one step restricted to the KAM_PHASE_* bits in `phases`, applied in
+ * the order advect, div+FFT, tridiagonal, inverse FFT + gradient.  Projection
+ * phases act on the current state in place. */
+int kam_sim_step_phases(kam_sim *sim, int phases);
+
+/* Copies the half spectrum buffer (float2 [M = nPhi/2][nTheta],
+ * mode-major; slot 0 holds (mode 0, Nyquist)).  After KAM_PHASE_DIVFFT it
+ * holds d-hat, after KAM_PHASE_TRI p-hat. */
+int kam_sim_get_spectrum(kam_sim *sim, float *spec);
+
+/* Runs nsteps full steps and times them with HIP events on the simulation's
+ * stream.  step_ms[nsteps] (optional) receives each step's duration,
+ * kernel_ms[4] (optional, needs per_kernel = 1) the mean per-step duration of
+ * the advection, div+FFT, tridiagonal and inverse-FFT+gradient kernels, and
+ * total_ms the whole interval.  per_kernel = 1 records events between the
+ * kernels and waits for each step; use 0 for throughput. */
+int kam_sim_benchmark(kam_sim *sim, int nsteps, int per_kernel, double *step_ms,
+                      double *kernel_ms, double *total_ms);
+
+/* Waits for all queued work of the simulation. */
+int kam_sim_sync(kam_sim *sim);
+
+/* Bytes of device memory one simulation of this shape needs. */
+size_t kam_sim_bytes(int n_theta, int n_phi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KAMINO_H */

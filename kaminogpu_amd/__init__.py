@@ -1,0 +1,200 @@
+"""kaminogpu_amd -- MI355X-native KaminoGPU time step.
+
+Python view of libkamino.so (include/kamino.h) for tests, benchmarks and
+scripting.  The compute path is the HIP library; if it is not built, importing
+`lib()` raises: there is no CPU fallback.
+
+The class `KaminoSolver` mirrors the reference's solver API
+(KaminoSolver.cuh:99-111): construct with the grid, set the fields,
+`stepForward`, read the fields back.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libkamino.so")
+
+KAM_OK = 0
+PHASE_ADVECT, PHASE_DIVFFT, PHASE_TRI, PHASE_IFFT = 1, 8, 16, 32
+PHASE_PROJECT = 8 | 16 | 32
+PHASE_ALL = 1 | 8 | 16 | 32
+PREC_FP32 = 0
+
+# every symbol include/kamino.h declares
+ABI_SYMBOLS = [
+    "kam_last_error", "kam_version", "kam_device_count",
+    "kam_sim_create", "kam_sim_destroy", "kam_sim_set_fields", "kam_sim_get_fields",
+    "kam_sim_step", "kam_sim_step_phases", "kam_sim_get_spectrum", "kam_sim_sync",
+    "kam_sim_bytes", "kam_sim_benchmark",
+]
+
+
+class KaminoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"kamino error {code}: {msg}")
+        self.code = code
+
+
+class SimDesc(C.Structure):
+    _fields_ = [
+        ("n_theta", C.c_int), ("n_phi", C.c_int), ("radius", C.c_double), ("dt", C.c_double),
+        ("geometric", C.c_int), ("precision", C.c_int), ("device", C.c_int),
+    ]
+
+
+_fp = C.POINTER(C.c_float)
+_lib = None
+
+
+def lib():
+    """Loads libkamino.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run python -m kaminogpu_amd.build")
+    L = C.CDLL(LIB_PATH)
+    L.kam_last_error.restype = C.c_char_p
+    L.kam_version.restype = C.c_char_p
+    L.kam_device_count.restype = C.c_int
+    L.kam_sim_create.argtypes = [C.POINTER(SimDesc), C.POINTER(C.c_void_p)]
+    L.kam_sim_destroy.argtypes = [C.c_void_p]
+    L.kam_sim_set_fields.argtypes = [C.c_void_p, _fp, _fp, _fp]
+    L.kam_sim_get_fields.argtypes = [C.c_void_p, _fp, _fp, _fp]
+    L.kam_sim_step.argtypes = [C.c_void_p, C.c_int]
+    L.kam_sim_step_phases.argtypes = [C.c_void_p, C.c_int]
+    L.kam_sim_get_spectrum.argtypes = [C.c_void_p, _fp]
+    L.kam_sim_sync.argtypes = [C.c_void_p]
+    _dpp = C.POINTER(C.c_double)
+    L.kam_sim_benchmark.argtypes = [C.c_void_p, C.c_int, C.c_int, _dpp, _dpp, _dpp]
+    L.kam_sim_bytes.restype = C.c_size_t
+    L.kam_sim_bytes.argtypes = [C.c_int, C.c_int]
+    _bind_engine(L)
+    _lib = L
+    return L
+
+
+def _bind_engine(L):
+    try:
+        from . import engine as _engine  # noqa: F401
+    except ImportError:
+        return
+    _engine.bind(L)
+
+
+def check(rc):
+    if rc != KAM_OK:
+        raise KaminoError(rc, lib().kam_last_error().decode())
+    return rc
+
+
+def _f32(a, shape):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.shape != shape:
+        raise ValueError(f"expected shape {shape}, got {a.shape}")
+    return a
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(_fp)
+
+
+def device_count() -> int:
+    return lib().kam_device_count()
+
+
+class Sim:
+    """One simulation on one GPU (the plain, one-sim-at-a-time form)."""
+
+    def __init__(self, n_theta, dt, radius=1.0, geometric=True, device=0, precision=PREC_FP32):
+        self.n_theta, self.n_phi = int(n_theta), 2 * int(n_theta)
+        self.desc = SimDesc(self.n_theta, self.n_phi, float(radius), float(dt), int(bool(geometric)),
+                            int(precision), int(device))
+        h = C.c_void_p()
+        check(lib().kam_sim_create(C.byref(self.desc), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().kam_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def shapes(self):
+        nt, nph = self.n_theta, self.n_phi
+        return (nt, nph), (nt - 1, nph), (nt, nph)
+
+    def set_fields(self, u_phi, u_theta, rho=None):
+        sp, st, sr = self.shapes
+        a, b = _f32(u_phi, sp), _f32(u_theta, st)
+        c = None if rho is None else _f32(rho, sr)
+        check(lib().kam_sim_set_fields(self._h, _ptr(a), _ptr(b), _ptr(c)))
+
+    def get_fields(self):
+        sp, st, sr = self.shapes
+        a, b, c = np.empty(sp, np.float32), np.empty(st, np.float32), np.empty(sr, np.float32)
+        check(lib().kam_sim_get_fields(self._h, _ptr(a), _ptr(b), _ptr(c)))
+        return a, b, c
+
+    def step(self, nsteps=1):
+        check(lib().kam_sim_step(self._h, int(nsteps)))
+
+    def step_phases(self, phases):
+        check(lib().kam_sim_step_phases(self._h, int(phases)))
+
+    def spectrum(self):
+        """Half spectrum, complex64 [M][nTheta] (mode-major; slot 0 = (mode 0, Nyquist))."""
+        M = self.n_phi // 2
+        s = np.empty((M, self.n_theta, 2), np.float32)
+        check(lib().kam_sim_get_spectrum(self._h, _ptr(s)))
+        return s[..., 0] + 1j * s[..., 1]
+
+    def sync(self):
+        check(lib().kam_sim_sync(self._h))
+
+    def benchmark(self, nsteps, per_kernel=False):
+        """Times nsteps steps with HIP events on the simulation's stream.
+        Returns (total_ms, step_ms[nsteps], kernel_ms[4] or None)."""
+        dp = C.POINTER(C.c_double)
+        steps = np.zeros(nsteps, np.float64)
+        kern = np.zeros(4, np.float64)
+        tot = C.c_double(0.0)
+        check(lib().kam_sim_benchmark(self._h, int(nsteps), int(bool(per_kernel)),
+                                      steps.ctypes.data_as(dp), kern.ctypes.data_as(dp), C.byref(tot)))
+        return tot.value, steps, (kern if per_kernel else None)
+
+
+class KaminoSolver(Sim):
+    """The reference's solver interface (KaminoSolver.cuh:99-111) over the C ABI.
+
+    KaminoSolver(nPhi, nTheta, radius, frameDuration, ...) -> here the
+    frame duration is not used by the step, and dt is fixed at construction as
+    in the reference, whose kernels read the constant timeStepGlobal
+    (KaminoCore.cu:871) and ignore stepForward's argument.
+    """
+
+    def __init__(self, n_phi, n_theta, radius, dt, geometric=True, device=0):
+        if n_phi != 2 * n_theta:
+            raise ValueError("nPhi must be 2 * nTheta")
+        super().__init__(n_theta, dt, radius=radius, geometric=geometric, device=device)
+        self.time_elapsed = 0.0
+
+    def stepForward(self, time_step=None):  # noqa: N802 -- reference name
+        self.step(1)
+        self.time_elapsed += self.desc.dt if time_step is None else time_step

@@ -1,0 +1,68 @@
+// kamino_device.h -- device-side data layout shared by the gfx950 kernels and
+// the host runtime.  Plain structs, no HIP types in the layout.
+//
+// HBM layout of one simulation (all FP32, dense theta-major rows, phi
+// contiguous; KaminoQuantity.cu:70-73 uses the same [theta][phi] order):
+//   state[b][0] u_phi   [nTheta][nPhi]      b = 0, 1 (ping-pong per step)
+//   state[b][1] u_theta [nTheta][nPhi]      (row nTheta-1 is padding, zeroed)
+//   state[b][2] rho     [nTheta][nPhi]
+//   spec        [M][nTheta] float2          half spectrum, mode-major, M = nPhi/2;
+//                                           slot 0 packs (mode 0, Nyquist mode)
+// 28 bytes per cell in total.  Per-size tables (row metrics, FP64 Poisson
+// rows, twiddles) are shared by every simulation of the same shape and dt.
+#pragma once
+
+#include <stdint.h>
+
+namespace kam {
+
+enum FieldIdx { F_UPHI = 0, F_UTHETA = 1, F_RHO = 2 };
+
+// Row tables, FP32, each nTheta long, packed one after another.
+enum RowTab {
+    RT_COFPHI_C = 0,  // dt / (R sin(theta_j) h), theta_j = (j + 1/2) h   (u_phi and rho nodes)
+    RT_COFPHI_T = 1,  // dt / (R sin(theta) h),  theta = (j + 1) h        (u_theta nodes)
+    RT_G = 2,         // dt cos(theta_j) / (R sin(theta_j))                (geometric)
+    RT_DIVS = 3,      // sin(theta_j + h/2) / (h sin theta_j)
+    RT_DIVN = 4,      // sin(theta_j - h/2) / (h sin theta_j)
+    RT_INVHS = 5,     // 1 / (h sin theta_j)
+    RT_COUNT = 6
+};
+
+// FP64 tables for the tridiagonal rows, each nTheta long.
+enum RowTabD {
+    RD_COT2H = 0,    // cot(theta_j) / (2 h)
+    RD_INVSIN2 = 1,  // 1 / sin^2(theta_j)
+    RD_COUNT = 2
+};
+
+enum SimFlags { SIM_GEOMETRIC = 1 };
+
+struct SimDesc {
+    int nTheta, nPhi, M;   // M = nPhi / 2
+    int flags;             // SimFlags
+    float cofTheta;        // dt / (R h): theta cells per unit u_theta
+    float invH;            // 1 / h
+    double invH2;          // 1 / h^2
+    float* state[2][3];    // [parity][field]
+    float* spec;           // float2 [M][nTheta]
+    const float* rowf;     // RT_COUNT * nTheta
+    const double* rowd;    // RD_COUNT * nTheta
+    const float* twN;      // float2 [nPhi]: exp(-2 pi i n / nPhi)
+    int nfac;              // radix factors of M, innermost first
+    int fac[12];
+    int advRows;           // rows per advection tile
+    int fftRows;           // rows per FFT tile
+    int pad0, pad1;
+};
+
+// One unit of work: rows [row0, row0 + rows) of simulation `sim` (or, for the
+// tridiagonal kernel, modes [row0, row0 + rows)).
+struct Tile {
+    int sim;
+    int row0;
+    int rows;
+    int pad;
+};
+
+}  // namespace kam

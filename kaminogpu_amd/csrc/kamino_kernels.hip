@@ -1,0 +1,769 @@
+// kamino_kernels.hip -- gfx950 kernels of the spherical-fluid time step.
+//
+// One step of a batch is four launches over a device table of simulations
+// (SimDesc) and a list of tiles:
+//   k_advect      RK2 semi-Lagrangian advection of u_phi, u_theta and rho with
+//                 the geometric (metric-term) update fused into its epilogue
+//   k_div_fft     divergence + real FFT along phi per latitude row
+//   k_tridiag     one tridiagonal solve in theta per Fourier mode
+//   k_ifft_grad   inverse real FFT + pressure-gradient subtraction
+// All coordinates are handled in cell-index space: an integer node index plus
+// a small float offset, so FP32 keeps its precision for the interpolation
+// weights however far the node sits from phi = 0.
+#include <hip/hip_runtime.h>
+
+#include "kamino_device.h"
+#include "kamino_launch.h"
+
+namespace kam {
+
+enum Kind { K_VPHI = 0, K_VTHETA = 1, K_CENTERED = 2 };
+
+// ---------------------------------------------------------------------------
+// Sampling (KaminoCore.cu:11-184).  The reference folds radian coordinates
+// over the poles (validateCoord, :11-29) and picks one of two bilinear
+// branches; this is the same map in index space.  `bj + fth` is the
+// offset-shifted theta coordinate in cells, `bi + fphi` the phi coordinate.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+__device__ __forceinline__ int wrap_phi(int i, int nPhi) {
+    if ((unsigned)i >= (unsigned)nPhi) {
+        i %= nPhi;
+        if (i < 0) i += nPhi;
+    }
+    return i;
+}
+
+template <int KIND>
+__device__ __forceinline__ float sample(const float* __restrict__ f, int nTheta, int nPhi, int bi,
+                                        float fphi, int bj, float fth) {
+    const int rows = (KIND == K_VTHETA) ? nTheta - 1 : nTheta;
+    const int poleRow = (KIND == K_VTHETA) ? nTheta - 2 : nTheta - 1;
+    const float kp = floorf(fphi);
+    int ip = bi + (int)kp;
+    const float ap = fphi - kp;
+    const float kt = floorf(fth);
+    int jt = bj + (int)kt;
+    float at = fth - kt;
+    // fold: t < 0 -> -t, and t > nTheta -> 2 nTheta - t, each adding half a turn in phi
+    bool flip = false;
+    if (jt < 0) {
+        flip = true;
+        if (at > 0.f) {
+            jt = -jt - 1;
+            at = 1.f - at;
+        } else {
+            jt = -jt;
+        }
+    } else if (jt > nTheta || (jt == nTheta && at > 0.f)) {
+        flip = true;
+        if (at > 0.f) {
+            jt = 2 * nTheta - jt - 1;
+            at = 1.f - at;
+        } else {
+            jt = 2 * nTheta - jt;
+        }
+    }
+    const int half = nPhi >> 1;
+    if (flip) ip += half;
+    ip = wrap_phi(ip, nPhi);
+    int ip1 = ip + 1;
+    if (ip1 == nPhi) ip1 = 0;
+    if ((jt == 0 && flip) || jt == poleRow) {
+        // pole branch: the same row at phi and phi + pi
+        const int r = min(max(jt, 0), rows - 1);
+        const float* row = f + (size_t)r * nPhi;
+        int ia = ip + half;
+        if (ia >= nPhi) ia -= nPhi;
+        int ib = ia + 1;
+        if (ib == nPhi) ib = 0;
+        const float lo = lerpf(row[ip], row[ip1], ap);
+        const float hi = lerpf(row[ia], row[ib], ap);
+        return lerpf(lo, hi, (KIND == K_VPHI) ? at : 0.5f * at);
+    }
+    const int r0 = min(max(jt, 0), rows - 1);
+    const int r1 = min(max(jt + 1, 0), rows - 1);
+    const float* row0 = f + (size_t)r0 * nPhi;
+    const float* row1 = f + (size_t)r1 * nPhi;
+    const float lo = lerpf(row0[ip], row0[ip1], ap);
+    const float hi = lerpf(row1[ip], row1[ip1], ap);
+    return lerpf(lo, hi, at);
+}
+
+// Node offsets of each grid in cells (KaminoHeader.cuh:30-41) and the shift
+// that maps a raw coordinate onto that grid's sample lattice.
+__device__ constexpr float kNodeP[3] = {-0.5f, 0.f, 0.f};
+__device__ constexpr float kNodeT[3] = {0.5f, 1.f, 0.5f};
+__device__ constexpr float kShiftP[3] = {0.5f, 0.f, 0.f};
+__device__ constexpr float kShiftT[3] = {-0.5f, -1.f, -0.5f};
+
+// RK2 backtrace from node (i, j) of grid NODE, sampling field `src` of grid
+// SRC at the departure point (KaminoCore.cu:186-319).  cofP converts u_phi to
+// phi cells per step at this node's latitude, cofT u_theta to theta cells.
+template <int NODE, int SRC>
+__device__ __forceinline__ float advect_node(const float* __restrict__ up,
+                                             const float* __restrict__ ut,
+                                             const float* __restrict__ src, int nTheta, int nPhi,
+                                             int i, int j, float cofP, float cofT) {
+    constexpr float pP = kNodeP[NODE] + kShiftP[K_VPHI], pT = kNodeT[NODE] + kShiftT[K_VPHI];
+    constexpr float tP = kNodeP[NODE] + kShiftP[K_VTHETA], tT = kNodeT[NODE] + kShiftT[K_VTHETA];
+    constexpr float sP = kNodeP[NODE] + kShiftP[SRC], sT = kNodeT[NODE] + kShiftT[SRC];
+    const float u0p = sample<K_VPHI>(up, nTheta, nPhi, i, pP, j, pT);
+    const float u0t = sample<K_VTHETA>(ut, nTheta, nPhi, i, tP, j, tT);
+    float dp = u0p * cofP;
+    float dt = u0t * cofT;
+    const float ump = sample<K_VPHI>(up, nTheta, nPhi, i, pP - 0.5f * dp, j, pT - 0.5f * dt);
+    const float umt = sample<K_VTHETA>(ut, nTheta, nPhi, i, tP - 0.5f * dp, j, tT - 0.5f * dt);
+    dp = 0.5f * (u0p + ump) * cofP;
+    dt = 0.5f * (u0t + umt) * cofT;
+    return sample<SRC>(src, nTheta, nPhi, i, sP - dp, j, sT - dt);
+}
+
+// ---------------------------------------------------------------------------
+// Geometric update (KaminoCore.cu:457-514): the real root x of
+// G^2 x^3 + (1 + G vbar) x - ubar = 0, or ubar when |G| <= 1e-7.  The
+// reference evaluates Cardano's formula in FP32, whose two terms cancel to
+// the small root (relative error ~1e-3 near the equator); when 1 + G vbar > 0
+// the cubic is monotone and Newton from ubar / (1 + G vbar) converges to the
+// same unique root to FP32 precision.  Otherwise (three real roots possible)
+// the reference's own branch choice is evaluated in FP64.
+// ---------------------------------------------------------------------------
+__device__ double ref_root3_pos(double x) {
+    double s = 1.;
+    while (x < 1.) {
+        x *= 8.;
+        s *= 0.5;
+    }
+    while (x > 8.) {
+        x *= 0.125;
+        s *= 2.;
+    }
+    double r = 1.5;
+    for (int k = 0; k < 6; ++k) r -= 1. / 3. * (r - x / (r * r));
+    return r * s;
+}
+
+__device__ __noinline__ float cardano_fp64(double G, double vb, double ub) {
+    const double cof = G * G;
+    const double B = (G * vb + 1.0) / cof;
+    const double Cc = -ub / cof;
+    const double q = -3.0 * B / 9.0;
+    const double r = 27.0 * Cc / 54.0;
+    const double r2 = r * r, q3 = q * q * q;
+    if (r2 <= q3 + (double)1e-7f) {
+        double t = r / sqrt(q3);
+        t = t < -1 ? -1 : (t > 1 ? 1 : t);
+        return (float)(-2.0 * sqrt(q) * cos(acos(t) / 3.0));
+    }
+    const double s = fabs(r) + sqrt(r2 - q3);
+    double A = -(s > 0 ? ref_root3_pos(s) : 0.0);
+    if (r < 0) A = -A;
+    const double Bq = (A == 0) ? 0 : q / A;
+    return (float)(A + Bq);
+}
+
+__device__ __forceinline__ float geo_root(float G, float vb, float ub) {
+    if (!(fabsf(G) > 1e-7f)) return ub;
+    const float b = fmaf(G, vb, 1.f);
+    if (b > 0.f) {
+        const float a = G * G;
+        float x = ub / b;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const float x2 = x * x;
+            const float f = fmaf(a * x2, x, fmaf(b, x, -ub));
+            const float fp = fmaf(3.f * a, x2, b);
+            x -= f / fp;
+        }
+        return x;
+    }
+    return cardano_fp64(G, vb, ub);
+}
+
+// ---------------------------------------------------------------------------
+// k_advect: one tile = rows [j0, j0 + R) of one simulation, full phi width.
+// With the geometric update on, the tile advects one extra u_phi row and two
+// extra u_theta rows into LDS (the update reads the rows either side), solves
+// the cubic per cell and restaggers (assignPhiKernel :517-534,
+// assignThetaKernel :537-549) before anything leaves the CU.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims,
+                                                const Tile* __restrict__ tiles, int ntiles,
+                                                int parity) {
+    extern __shared__ float lds[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const SimDesc& s = sims[tile.sim];
+        const int nT = s.nTheta, nP = s.nPhi;
+        const float* __restrict__ up = s.state[parity][F_UPHI];
+        const float* __restrict__ ut = s.state[parity][F_UTHETA];
+        const float* __restrict__ rh = s.state[parity][F_RHO];
+        float* __restrict__ oup = s.state[parity ^ 1][F_UPHI];
+        float* __restrict__ out = s.state[parity ^ 1][F_UTHETA];
+        float* __restrict__ orh = s.state[parity ^ 1][F_RHO];
+        const float* __restrict__ cofPC = s.rowf + RT_COFPHI_C * nT;
+        const float* __restrict__ cofPT = s.rowf + RT_COFPHI_T * nT;
+        const float cofT = s.cofTheta;
+        const int j0 = tile.row0, R = tile.rows;
+
+        if (!(s.flags & SIM_GEOMETRIC)) {
+            for (int idx = tid; idx < R * nP; idx += nthr) {
+                const int r = idx / nP, i = idx - r * nP, j = j0 + r;
+                const size_t o = (size_t)j * nP + i;
+                oup[o] = advect_node<K_VPHI, K_VPHI>(up, ut, up, nT, nP, i, j, cofPC[j], cofT);
+                if (j < nT - 1)
+                    out[o] = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, nT, nP, i, j, cofPT[j], cofT);
+                orh[o] = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, nT, nP, i, j, cofPC[j], cofT);
+            }
+            continue;
+        }
+
+        const int uLo = j0, uHi = min(j0 + R, nT - 1);
+        const int vLo = max(j0 - 1, 0), vHi = min(j0 + R, nT - 2);
+        float* U = lds;                      // advected u_phi rows uLo..uHi
+        float* V = U + (size_t)(R + 1) * nP;  // advected u_theta rows vLo..vHi
+        float* X = V + (size_t)(R + 2) * nP;  // cubic roots x, rows j0..j0+R-1
+        const int nU = (uHi - uLo + 1) * nP, nV = (vHi - vLo + 1) * nP;
+        for (int idx = tid; idx < nU; idx += nthr) {
+            const int r = idx / nP, i = idx - r * nP, j = uLo + r;
+            U[idx] = advect_node<K_VPHI, K_VPHI>(up, ut, up, nT, nP, i, j, cofPC[j], cofT);
+        }
+        for (int idx = tid; idx < nV; idx += nthr) {
+            const int r = idx / nP, i = idx - r * nP, j = vLo + r;
+            V[idx] = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, nT, nP, i, j, cofPT[j], cofT);
+        }
+        for (int idx = tid; idx < R * nP; idx += nthr) {
+            const int r = idx / nP, i = idx - r * nP, j = j0 + r;
+            orh[(size_t)j * nP + i] =
+                advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, nT, nP, i, j, cofPC[j], cofT);
+        }
+        __syncthreads();
+
+        // geometricFillKernel per cell, walking down each owned column so that
+        // v' of the row above stays in a register for the u_theta restagger
+        const float* __restrict__ Gt = s.rowf + RT_G * nT;
+        const int half = nP >> 1;
+        for (int i = tid; i < nP; i += nthr) {
+            const int ir = (i + 1 == nP) ? 0 : i + 1;
+            const int io = (i + half) % nP;
+            float vprev = 0.f;
+            for (int j = j0; j <= uHi; ++j) {
+                const float* Ur = U + (size_t)(j - uLo) * nP;
+                const float ub = 0.5f * (Ur[i] + Ur[ir]);
+                float vb;
+                if (j == 0) {
+                    const float* V0 = V + (size_t)(0 - vLo) * nP;
+                    vb = 0.75f * V0[i] + 0.25f * V0[io];
+                } else if (j == nT - 1) {
+                    const float* Vl = V + (size_t)(nT - 2 - vLo) * nP;
+                    vb = 0.75f * Vl[i] + 0.25f * Vl[io];
+                } else {
+                    vb = 0.5f * (V[(size_t)(j - 1 - vLo) * nP + i] + V[(size_t)(j - vLo) * nP + i]);
+                }
+                const float G = Gt[j];
+                const float x = geo_root(G, vb, ub);
+                const float vp = fmaf(G * x, x, vb);
+                if (j < j0 + R) X[(size_t)(j - j0) * nP + i] = x;
+                if (j > j0) out[(size_t)(j - 1) * nP + i] = 0.5f * (vprev + vp);
+                vprev = vp;
+            }
+        }
+        __syncthreads();
+        for (int idx = tid; idx < R * nP; idx += nthr) {
+            const int r = idx / nP, i = idx - r * nP;
+            const int il = (i == 0) ? nP - 1 : i - 1;
+            oup[(size_t)(j0 + r) * nP + i] = 0.5f * (X[(size_t)r * nP + il] + X[idx]);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row FFT: Stockham autosort passes over rows held in LDS, radix 2/3/4/5,
+// every thread doing whole butterflies; a pass reads its inputs into
+// registers, synchronises, then writes its outputs in natural order.
+// DIR = -1 for exp(-i ...), +1 for exp(+i ...); unnormalised.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// DIR * i * a
+template <int DIR>
+__device__ __forceinline__ float2 cmuli(float2 a) {
+    return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+template <int RADIX, int DIR>
+__device__ __forceinline__ void butterfly(float2* x) {
+    if constexpr (RADIX == 2) {
+        const float2 a = x[0], b = x[1];
+        x[0] = cadd(a, b);
+        x[1] = csub(a, b);
+    } else if constexpr (RADIX == 3) {
+        const float2 t = cadd(x[1], x[2]), u = csub(x[1], x[2]);
+        const float2 m = csub(x[0], cscale(t, 0.5f));
+        const float2 n = cmuli<DIR>(cscale(u, 0.86602540378443864676f));
+        x[0] = cadd(x[0], t);
+        x[1] = cadd(m, n);
+        x[2] = csub(m, n);
+    } else if constexpr (RADIX == 4) {
+        const float2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+        const float2 c = cadd(x[1], x[3]), d = cmuli<DIR>(csub(x[1], x[3]));
+        x[0] = cadd(a, c);
+        x[2] = csub(a, c);
+        x[1] = cadd(b, d);
+        x[3] = csub(b, d);
+    } else {  // 5
+        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+        const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+        const float2 t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]);
+        const float2 t3 = csub(x[1], x[4]), t4 = csub(x[2], x[3]);
+        const float2 a1 = cadd(x[0], cadd(cscale(t1, c1), cscale(t2, c2)));
+        const float2 a2 = cadd(x[0], cadd(cscale(t1, c2), cscale(t2, c1)));
+        const float2 b1 = cmuli<DIR>(cadd(cscale(t3, s1), cscale(t4, s2)));
+        const float2 b2 = cmuli<DIR>(csub(cscale(t3, s2), cscale(t4, s1)));
+        x[0] = cadd(x[0], cadd(t1, t2));
+        x[1] = cadd(a1, b1);
+        x[4] = csub(a1, b1);
+        x[2] = cadd(a2, b2);
+        x[3] = csub(a2, b2);
+    }
+}
+
+constexpr int kFftMaxVals = 32;  // complex values per thread per pass: rows * M <= 32 * 256
+
+template <int RADIX, int DIR>
+__device__ void fft_pass(float2* buf, int nrows, int M, int Ls, const float2* __restrict__ twN,
+                         int twScale) {
+    constexpr int MAXB = kFftMaxVals / RADIX;
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int nb = M / RADIX;
+    const int total = nrows * nb;
+    float2 v[MAXB][RADIX];
+    int dst[MAXB];
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+        const int b = tid + c * nthr;
+        dst[c] = -1;
+        if (b < total) {
+            const int row = b / nb, bb = b - row * nb;
+            const int k = bb / Ls, j = bb - k * Ls;
+            const float2* base = buf + (size_t)row * M + j + k * Ls;
+            float2 x[RADIX];
+#pragma unroll
+            for (int q = 0; q < RADIX; ++q) {
+                float2 a = base[q * nb];
+                if (q > 0) {
+                    float2 w = twN[q * j * twScale];
+                    if (DIR > 0) w.y = -w.y;
+                    a = cmul(a, w);
+                }
+                x[q] = a;
+            }
+            butterfly<RADIX, DIR>(x);
+#pragma unroll
+            for (int m = 0; m < RADIX; ++m) v[c][m] = x[m];
+            dst[c] = row * M + j + k * Ls * RADIX;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+        if (dst[c] >= 0) {
+            const int Lsx = Ls;
+#pragma unroll
+            for (int m = 0; m < RADIX; ++m) buf[dst[c] + m * Lsx] = v[c][m];
+        }
+    }
+    __syncthreads();
+}
+
+template <int DIR>
+__device__ void fft_rows(float2* buf, int nrows, const SimDesc& s) {
+    const int M = s.M;
+    int Ls = 1;
+    for (int p = 0; p < s.nfac; ++p) {
+        const int r = s.fac[p];
+        const int twScale = s.nPhi / (Ls * r);
+        const float2* tw = reinterpret_cast<const float2*>(s.twN);
+        switch (r) {
+            case 2: fft_pass<2, DIR>(buf, nrows, M, Ls, tw, twScale); break;
+            case 3: fft_pass<3, DIR>(buf, nrows, M, Ls, tw, twScale); break;
+            case 4: fft_pass<4, DIR>(buf, nrows, M, Ls, tw, twScale); break;
+            default: fft_pass<5, DIR>(buf, nrows, M, Ls, tw, twScale); break;
+        }
+        Ls *= r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_div_fft: divergence (fillDivergenceKernel, KaminoCore.cu:587-638) of R
+// rows, packed two reals per complex (z_k = d_2k + i d_2k+1), FFT of length
+// M, split into the half spectrum X_n = E_n + exp(-i pi n / M) O_n and scaled
+// by 1/nPhi.  Mode n of row j goes to spec[n][j] (mode-major); slot 0 holds
+// (d-hat_0, d-hat_Nyquist).  This is the reference's cuFFT CUFFT_INVERSE
+// call plus shiftFKernel (:640-656): d-hat_n = (1/nPhi) sum_i d_i e^{-i n i h}.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sims,
+                                                 const Tile* __restrict__ tiles, int ntiles,
+                                                 int parity) {
+    extern __shared__ float2 zbuf[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const SimDesc& s = sims[tile.sim];
+        const int nT = s.nTheta, nP = s.nPhi, M = s.M;
+        const int j0 = tile.row0, R = tile.rows;
+        const float* __restrict__ up = s.state[parity][F_UPHI];
+        const float* __restrict__ ut = s.state[parity][F_UTHETA];
+        const float* __restrict__ divS = s.rowf + RT_DIVS * nT;
+        const float* __restrict__ divN = s.rowf + RT_DIVN * nT;
+        const float* __restrict__ invHs = s.rowf + RT_INVHS * nT;
+        for (int idx = tid; idx < R * M; idx += nthr) {
+            const int r = idx / M, k = idx - r * M, j = j0 + r;
+            const int i0 = 2 * k, i1 = i0 + 1, i2 = (i1 + 1 == nP) ? 0 : i1 + 1;
+            const float* urow = up + (size_t)j * nP;
+            const float uw0 = urow[i0], ue0 = urow[i1], ue1 = urow[i2];
+            float vn0 = 0.f, vn1 = 0.f, vs0 = 0.f, vs1 = 0.f;
+            if (j > 0) {
+                vn0 = ut[(size_t)(j - 1) * nP + i0];
+                vn1 = ut[(size_t)(j - 1) * nP + i1];
+            }
+            if (j < nT - 1) {
+                vs0 = ut[(size_t)j * nP + i0];
+                vs1 = ut[(size_t)j * nP + i1];
+            }
+            const float fs = divS[j], fn = divN[j], fi = invHs[j];
+            const float d0 = (fs * vs0 - fn * vn0) + fi * (ue0 - uw0);
+            const float d1 = (fs * vs1 - fn * vn1) + fi * (ue1 - ue0);
+            zbuf[idx] = make_float2(d0, d1);
+        }
+        __syncthreads();
+        fft_rows<-1>(zbuf, R, s);
+        float2* __restrict__ spec = reinterpret_cast<float2*>(s.spec);
+        const float2* __restrict__ tw = reinterpret_cast<const float2*>(s.twN);
+        const float scale = 1.0f / (float)nP;
+        for (int idx = tid; idx < R * M; idx += nthr) {
+            const int n = idx / R, r = idx - n * R, j = j0 + r;
+            const float2 Zn = zbuf[(size_t)r * M + n];
+            float2 out;
+            if (n == 0) {
+                out = make_float2((Zn.x + Zn.y) * scale, (Zn.x - Zn.y) * scale);
+            } else {
+                const float2 Zm = zbuf[(size_t)r * M + (M - n)];
+                const float2 E = make_float2(0.5f * (Zn.x + Zm.x), 0.5f * (Zn.y - Zm.y));
+                const float2 O = make_float2(0.5f * (Zn.y + Zm.y), -0.5f * (Zn.x - Zm.x));
+                const float2 X = cadd(E, cmul(tw[n], O));
+                out = cscale(X, scale);
+            }
+            spec[(size_t)n * nT + j] = out;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_tridiag: T_n p-hat = d-hat for every Fourier mode n (crKernel, tdm.cu:3-96,
+// with the rows of precomputeABCKernel, KaminoSolver.cu:117-163: sub-diagonal
+// a = 1/h^2 - cot/(2h), diagonal b = -2/h^2 - n^2/sin^2, super-diagonal
+// c = 1/h^2 + cot/(2h), pole rows folded b += a / b += c).  Arithmetic is FP64.
+//
+// L lanes share one mode; lane l owns K consecutive rows and runs Thomas on
+// them in registers, keeping each row in the form
+//     alpha_i x_{s-1} + x_i + gamma_i x_{i+1} = delta_i
+// (s = the lane's first row).  With P/Q/S such that x_s = P + Q x_{s-1} + S x_e,
+// the lanes' last unknowns X_l = x_e satisfy one tridiagonal system across the
+// L lanes, solved by parallel cyclic reduction with cross-lane shuffles; every
+// lane then back-substitutes its own rows.  Mode 0 is the identity in the
+// reference and its result is discarded (cacheZeroComponents :672-680 and
+// shiftUKernel :682-703 subtract it); its slot carries the Nyquist mode.
+// ---------------------------------------------------------------------------
+template <int L>
+__device__ __forceinline__ double shfl_grp(double v, int srcLaneInGroup, int laneBase) {
+    return __shfl(v, laneBase + srcLaneInGroup, 64);
+}
+
+template <int L, int KMAX>
+__global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sims,
+                                                 const Tile* __restrict__ tiles, int ntiles) {
+    constexpr int G = 256 / L;  // modes per block
+    const int tid = threadIdx.x;
+    const int l = tid % L;               // lane within the mode group
+    const int g = tid / L;               // mode within the block
+    const int wl = threadIdx.x & 63;     // lane within the wavefront
+    const int base = wl - l;             // first wavefront lane of this group
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const SimDesc& s = sims[tile.sim];
+        const int nT = s.nTheta;
+        const bool active = g < tile.rows;
+        const int n = tile.row0 + (active ? g : 0);
+        const int K = (nT + L - 1) / L;
+        const int r0 = l * K;
+        int cnt = nT - r0;
+        cnt = cnt < 0 ? 0 : (cnt > K ? K : cnt);
+        if (!active) cnt = 0;
+        float2* __restrict__ col = reinterpret_cast<float2*>(s.spec) + (size_t)n * nT;
+        const double* __restrict__ cot2h = s.rowd + RD_COT2H * nT;
+        const double* __restrict__ isin2 = s.rowd + RD_INVSIN2 * nT;
+        const double h2 = s.invH2;
+        const double nn = (n == 0) ? (double)s.M * (double)s.M : (double)n * (double)n;
+
+        double al[KMAX], ga[KMAX], dr[KMAX], di[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            al[k] = 0.0;
+            ga[k] = 0.0;
+            dr[k] = 0.0;
+            di[k] = 0.0;
+        }
+        // forward elimination within the lane's rows
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            if (k < cnt) {
+                const int i = r0 + k;
+                const float2 d = col[i];
+                const double dre = (n == 0) ? 0.0 : (double)d.x;
+                const double dim = (double)d.y;
+                double a = h2 - cot2h[i], c = h2 + cot2h[i];
+                double b = -2.0 * h2 - nn * isin2[i];
+                if (i == 0) {
+                    b += a;
+                    a = 0.0;
+                }
+                if (i == nT - 1) {
+                    b += c;
+                    c = 0.0;
+                }
+                if (k == 0) {
+                    const double inv = 1.0 / b;
+                    al[k] = a * inv;
+                    ga[k] = c * inv;
+                    dr[k] = dre * inv;
+                    di[k] = dim * inv;
+                } else {
+                    const double inv = 1.0 / (b - a * ga[k - 1]);
+                    al[k] = -a * al[k - 1] * inv;
+                    ga[k] = c * inv;
+                    dr[k] = (dre - a * dr[k - 1]) * inv;
+                    di[k] = (dim - a * di[k - 1]) * inv;
+                }
+            }
+        }
+        // x_s = P + Q x_{s-1} + S x_e, by backward recurrence from the last row
+        double Pr = 0.0, Pi = 0.0, Q = 0.0, S = 1.0;
+#pragma unroll
+        for (int k = KMAX - 2; k >= 0; --k) {
+            if (k < cnt - 1) {
+                Pr = dr[k] - ga[k] * Pr;
+                Pi = di[k] - ga[k] * Pi;
+                Q = -al[k] - ga[k] * Q;
+                S = -ga[k] * S;
+            }
+        }
+        // reduced equation of this lane: A X_{l-1} + B X_l + C X_{l+1} = D
+        const int src = (l + 1 < L) ? l + 1 : l;
+        const double nPr = shfl_grp<L>(Pr, src, base), nPi = shfl_grp<L>(Pi, src, base);
+        const double nQ = shfl_grp<L>(Q, src, base), nS = shfl_grp<L>(S, src, base);
+        double A = 0.0, B = 1.0, C = 0.0, Dr = 0.0, Di = 0.0;
+        if (cnt > 0) {
+            double gE = 0.0, aE = 0.0, drE = 0.0, diE = 0.0;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k)
+                if (k == cnt - 1) {
+                    gE = ga[k];
+                    aE = al[k];
+                    drE = dr[k];
+                    diE = di[k];
+                }
+            const bool hasNext = (l + 1 < L);
+            A = aE;
+            B = 1.0 + (hasNext ? gE * nQ : 0.0);
+            C = hasNext ? gE * nS : 0.0;
+            Dr = drE - (hasNext ? gE * nPr : 0.0);
+            Di = diE - (hasNext ? gE * nPi : 0.0);
+        }
+        // parallel cyclic reduction across the L lanes of the group
+#pragma unroll
+        for (int st = 1; st < L; st <<= 1) {
+            const int lm = l - st, lp = l + st;
+            const int sm = lm >= 0 ? lm : l, sp = lp < L ? lp : l;
+            double Am = shfl_grp<L>(A, sm, base), Bm = shfl_grp<L>(B, sm, base);
+            double Cm = shfl_grp<L>(C, sm, base), Dmr = shfl_grp<L>(Dr, sm, base);
+            double Dmi = shfl_grp<L>(Di, sm, base);
+            double Ap = shfl_grp<L>(A, sp, base), Bp = shfl_grp<L>(B, sp, base);
+            double Cp = shfl_grp<L>(C, sp, base), Dpr = shfl_grp<L>(Dr, sp, base);
+            double Dpi = shfl_grp<L>(Di, sp, base);
+            if (lm < 0) {
+                Am = 0.0;
+                Bm = 1.0;
+                Cm = 0.0;
+                Dmr = 0.0;
+                Dmi = 0.0;
+            }
+            if (lp >= L) {
+                Ap = 0.0;
+                Bp = 1.0;
+                Cp = 0.0;
+                Dpr = 0.0;
+                Dpi = 0.0;
+            }
+            const double k1 = A / Bm, k2 = C / Bp;
+            const double nA = -Am * k1, nC = -Cp * k2;
+            const double nB = B - Cm * k1 - Ap * k2;
+            const double nDr = Dr - Dmr * k1 - Dpr * k2;
+            const double nDi = Di - Dmi * k1 - Dpi * k2;
+            A = nA;
+            B = nB;
+            C = nC;
+            Dr = nDr;
+            Di = nDi;
+        }
+        const double Xr = Dr / B, Xi = Di / B;
+        const int sprev = (l > 0) ? l - 1 : l;
+        double xmr = shfl_grp<L>(Xr, sprev, base), xmi = shfl_grp<L>(Xi, sprev, base);
+        if (l == 0) {
+            xmr = 0.0;
+            xmi = 0.0;
+        }
+        // back substitution: x_i = delta_i - alpha_i x_{s-1} - gamma_i x_{i+1}
+        double xr = Xr, xi = Xi;
+#pragma unroll
+        for (int k = KMAX - 1; k >= 0; --k) {
+            if (k < cnt) {
+                if (k < cnt - 1) {
+                    xr = dr[k] - al[k] * xmr - ga[k] * xr;
+                    xi = di[k] - al[k] * xmi - ga[k] * xi;
+                }
+                col[r0 + k] = make_float2((n == 0) ? 0.f : (float)xr, (float)xi);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_ifft_grad: pressure rows j0..j0+R (one halo row) from the solved half
+// spectrum by a length-M inverse FFT of
+//   Y_n = (P_n + conj P_{M-n}) + i e^{i pi n / M} (P_n - conj P_{M-n}),
+// whose output y_k = p_2k + i p_2k+1 is the real pressure row in natural order
+// (cuFFT CUFFT_FORWARD plus shiftUKernel, KaminoCore.cu:682-703, with the
+// n = 0 mode dropped), then the gradient subtraction of applyPressureTheta
+// (:705-722) and applyPressurePhi (:724-747), in place.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ sims,
+                                                   const Tile* __restrict__ tiles, int ntiles,
+                                                   int parity) {
+    extern __shared__ float2 zbuf[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const Tile tile = tiles[t];
+        const SimDesc& s = sims[tile.sim];
+        const int nT = s.nTheta, nP = s.nPhi, M = s.M;
+        const int j0 = tile.row0, R = tile.rows;
+        const int Rp = min(R + 1, nT - j0);  // rows of p needed
+        const float2* __restrict__ spec = reinterpret_cast<const float2*>(s.spec);
+        const float2* __restrict__ tw = reinterpret_cast<const float2*>(s.twN);
+        for (int idx = tid; idx < Rp * M; idx += nthr) {
+            const int n = idx / Rp, r = idx - n * Rp, j = j0 + r;
+            const float2 Pn = spec[(size_t)n * nT + j];
+            float2 Y;
+            if (n == 0) {
+                Y = make_float2(Pn.y, -Pn.y);  // P_0 = 0 (mode dropped), P_M = Nyquist
+            } else {
+                const float2 Pm = spec[(size_t)(M - n) * nT + j];
+                const float2 ev = make_float2(Pn.x + Pm.x, Pn.y - Pm.y);
+                const float2 od = make_float2(Pn.x - Pm.x, Pn.y + Pm.y);
+                float2 w = tw[n];
+                w.y = -w.y;  // exp(+i pi n / M)
+                const float2 o = cmul(w, od);
+                Y = make_float2(ev.x - o.y, ev.y + o.x);
+            }
+            zbuf[(size_t)r * M + n] = Y;
+        }
+        __syncthreads();
+        fft_rows<+1>(zbuf, Rp, s);
+        const float* p = reinterpret_cast<const float*>(zbuf);  // p[r][i], row stride nP
+        float* __restrict__ up = s.state[parity][F_UPHI];
+        float* __restrict__ ut = s.state[parity][F_UTHETA];
+        const float* __restrict__ invHs = s.rowf + RT_INVHS * nT;
+        const float invH = s.invH;
+        for (int idx = tid; idx < R * nP; idx += nthr) {
+            const int r = idx / nP, i = idx - r * nP, j = j0 + r;
+            const int il = (i == 0) ? nP - 1 : i - 1;
+            const float pc = p[(size_t)r * nP + i];
+            const size_t o = (size_t)j * nP + i;
+            up[o] -= (pc - p[(size_t)r * nP + il]) * invHs[j];
+            if (j < nT - 1) ut[o] -= (p[(size_t)(r + 1) * nP + i] - pc) * invH;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+static inline int grid_for(int ntiles, int cap) { return ntiles < cap ? ntiles : cap; }
+
+hipError_t launch_advect(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
+                         size_t ldsBytes, int gridCap, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_advect, dim3(grid_for(ntiles, gridCap)), dim3(256), ldsBytes, st, sims,
+                       tiles, ntiles, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
+                          size_t ldsBytes, int gridCap, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_div_fft, dim3(grid_for(ntiles, gridCap)), dim3(256), ldsBytes, st, sims,
+                       tiles, ntiles, parity);
+    return hipGetLastError();
+}
+
+hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles,
+                          int ntiles, int gridCap, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    const dim3 grid(grid_for(ntiles, gridCap)), block(256);
+#define KAM_TRI(LL, KK)                                                                 \
+    if (lanesPerMode == LL && kmax == KK) {                                             \
+        hipLaunchKernelGGL((k_tridiag<LL, KK>), grid, block, 0, st, sims, tiles, ntiles); \
+        return hipGetLastError();                                                       \
+    }
+    KAM_TRI(8, 8)
+    KAM_TRI(16, 8)
+    KAM_TRI(32, 8)
+    KAM_TRI(64, 8)
+    KAM_TRI(64, 16)
+#undef KAM_TRI
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
+                            size_t ldsBytes, int gridCap, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ifft_grad, dim3(grid_for(ntiles, gridCap)), dim3(256), ldsBytes, st, sims,
+                       tiles, ntiles, parity);
+    return hipGetLastError();
+}
+
+}  // namespace kam
+
+namespace kam {
+
+hipError_t configure_kernels(size_t bytes) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_advect,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_div_fft, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void*)k_ifft_grad,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+}  // namespace kam

@@ -1,0 +1,94 @@
+// kamino_runtime.h -- host runtime shared by the single-simulation path and the
+// batching / serving engine: shape plans, per-shape device tables, and device
+// batches (descriptor table + tile lists + launch sequence).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "kamino.h"
+#include "kamino_device.h"
+
+namespace kam {
+
+// Records the message for kam_last_error() and returns `code`.
+int set_err(int code, const char* fmt, ...);
+
+#define KAM_HIP(call)                                                               \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess)                                                       \
+            return ::kam::set_err(KAM_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// Launch geometry of one simulation shape.
+struct Plan {
+    int nT = 0, nP = 0, M = 0;
+    int nfac = 0;
+    int fac[12] = {0};
+    int advRows = 0;   // rows per advection tile
+    int fftRows = 0;   // rows per FFT tile
+    int triL = 0;      // lanes per mode in the tridiagonal kernel
+    int triK = 0;      // template row capacity per lane
+    size_t advLds = 0; // bytes of LDS of an advection tile (geometric on)
+    size_t fftLds = 0; // bytes of LDS of an FFT tile (forward or inverse)
+};
+
+// Validates a shape and fills its plan; returns KAM_OK or KAM_EINVAL.
+int make_plan(int nTheta, int nPhi, Plan& plan);
+// Overrides the tile heights (0 keeps the default); used by the autotuner.
+int adjust_plan(Plan& plan, int advRows, int fftRows);
+
+// Device tables shared by every simulation of one (device, shape, dt, R).
+struct Tables {
+    float* rowf = nullptr;
+    double* rowd = nullptr;
+    float* twN = nullptr;
+};
+int acquire_tables(int device, int nTheta, double dt, double radius, Tables* out);
+void release_tables(int device, int nTheta, double dt, double radius);
+
+// Fills the host descriptor of a simulation whose state lives at `mem`
+// (7 * nTheta * nPhi floats: two state copies of three fields, then the
+// spectrum).
+void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& sd, float* mem);
+size_t sim_floats(int nTheta, int nPhi);
+
+// The launch sequence of one step over a set of simulations.
+struct TriClass {
+    int L, K, off, count;
+};
+
+class DeviceBatch {
+public:
+    ~DeviceBatch();
+    // Rebuilds tile lists for these simulations and uploads descriptors and
+    // tiles on `st` (synchronously with respect to the host buffers).
+    int build(const std::vector<SimDesc>& sims, const std::vector<Plan>& plans, hipStream_t st);
+    // Queues the phases of one step.  `parity` selects the current state copy;
+    // returns the parity after the step (flipped iff the advection ran).
+    int launch(int phases, int parity, hipStream_t st, int* newParity,
+               hipEvent_t* kernelEvents = nullptr);
+    int nsims() const { return nsims_; }
+    size_t capacityBytes() const { return capSims_ * sizeof(SimDesc) + capTiles_ * sizeof(Tile); }
+
+private:
+    int reserve(size_t nsims, size_t ntiles);
+    SimDesc* dSims_ = nullptr;
+    Tile* dTiles_ = nullptr;
+    size_t capSims_ = 0, capTiles_ = 0;
+    std::vector<Tile> hTiles_;
+    int nsims_ = 0;
+    int advOff_ = 0, nAdv_ = 0, fftOff_ = 0, nFft_ = 0;
+    std::vector<TriClass> tri_;
+    size_t advLds_ = 0, fftLds_ = 0;
+};
+
+// Kernel-time categories reported by profiling.
+enum KernelCat { KC_ADVECT = 0, KC_DIVFFT = 1, KC_TRI = 2, KC_IFFT = 3, KC_COUNT = 4 };
+
+int ensure_device(int device);
+
+}  // namespace kam

@@ -1,0 +1,548 @@
+// kamino_runtime.hip -- host runtime: shape plans, per-shape tables, device
+// batches and the single-simulation C ABI (kam_sim_*).
+#include "kamino_runtime.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "kamino_launch.h"
+
+namespace kam {
+
+static thread_local std::string g_err;
+
+int set_err(int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+// Reference constants, KaminoHeader.cuh:26-27.
+static const double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Plans
+// ---------------------------------------------------------------------------
+static int factor_m(int M, int* fac) {
+    int n = 0;
+    while (M % 4 == 0 && M > 4 && n < 12) {
+        fac[n++] = 4;
+        M /= 4;
+    }
+    const int primes[4] = {4, 2, 3, 5};
+    for (int p : primes) {
+        while (M % p == 0 && M > 1 && n < 12) {
+            fac[n++] = p;
+            M /= p;
+        }
+    }
+    return M == 1 ? n : -1;
+}
+
+static size_t adv_lds(int R, int nP) { return (size_t)(3 * R + 3) * nP * sizeof(float); }
+static size_t fft_lds(int R, int M) { return (size_t)(R + 1) * M * 2 * sizeof(float); }
+
+int make_plan(int nT, int nP, Plan& p) {
+    if (nT < 4) return set_err(KAM_EINVAL, "n_theta must be >= 4 (got %d)", nT);
+    if (nP != 2 * nT) return set_err(KAM_EINVAL, "n_phi must equal 2*n_theta (got %d, %d)", nP, nT);
+    if (nT > 1024) return set_err(KAM_EINVAL, "n_theta must be <= 1024 (got %d)", nT);
+    p = Plan();
+    p.nT = nT;
+    p.nP = nP;
+    p.M = nP / 2;
+    p.nfac = factor_m(p.M, p.fac);
+    if (p.nfac < 0)
+        return set_err(KAM_EINVAL, "n_theta must factor as 2^a 3^b 5^c (got %d)", nT);
+    // advection tile: LDS (3R+3) rows of nPhi floats, kept to 64 KiB where possible
+    const size_t cap = nP <= 1024 ? 64 * 1024 : 150 * 1024;
+    int R = (int)(cap / (12 * (size_t)nP)) - 1;
+    R = R < 1 ? 1 : (R > 8 ? 8 : R);
+    p.advRows = R > nT ? nT : R;
+    // FFT tile: R * M <= 4096 complex, plus one halo row for the inverse
+    int Rf = 4096 / p.M;
+    Rf = Rf < 1 ? 1 : Rf;
+    p.fftRows = Rf > nT ? nT : Rf;
+    if ((size_t)(p.fftRows + 1) * p.M > 7680) p.fftRows = 7680 / p.M - 1;
+    if (p.fftRows < 1) return set_err(KAM_EINVAL, "n_phi %d too large for the row FFT", nP);
+    // tridiagonal: L lanes per mode, at most K rows per lane
+    if (nT <= 64) { p.triL = 8; p.triK = 8; }
+    else if (nT <= 128) { p.triL = 16; p.triK = 8; }
+    else if (nT <= 256) { p.triL = 32; p.triK = 8; }
+    else if (nT <= 512) { p.triL = 64; p.triK = 8; }
+    else { p.triL = 64; p.triK = 16; }
+    p.advLds = adv_lds(p.advRows, nP);
+    p.fftLds = fft_lds(p.fftRows, p.M);
+    return KAM_OK;
+}
+
+int adjust_plan(Plan& p, int advRows, int fftRows) {
+    if (advRows > 0) {
+        if (adv_lds(advRows, p.nP) > 160 * 1024) return set_err(KAM_EINVAL, "advection tile too tall");
+        p.advRows = advRows > p.nT ? p.nT : advRows;
+        p.advLds = adv_lds(p.advRows, p.nP);
+    }
+    if (fftRows > 0) {
+        if ((size_t)(fftRows + 1) * p.M > 7680) return set_err(KAM_EINVAL, "FFT tile too tall");
+        p.fftRows = fftRows > p.nT ? p.nT : fftRows;
+        p.fftLds = fft_lds(p.fftRows, p.M);
+    }
+    return KAM_OK;
+}
+
+size_t sim_floats(int nT, int nP) { return (size_t)7 * nT * nP; }
+
+// ---------------------------------------------------------------------------
+// Per-shape tables
+// ---------------------------------------------------------------------------
+struct TabEntry {
+    Tables t;
+    int refs = 0;
+};
+static std::mutex g_tab_mu;
+static std::map<std::tuple<int, int, double, double>, TabEntry> g_tabs;
+
+int acquire_tables(int device, int nT, double dt, double radius, Tables* out) {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto key = std::make_tuple(device, nT, dt, radius);
+    auto it = g_tabs.find(key);
+    if (it != g_tabs.end()) {
+        it->second.refs++;
+        *out = it->second.t;
+        return KAM_OK;
+    }
+    const int nP = 2 * nT;
+    const double h = kPi / nT;  // KaminoCore.cu:849
+    std::vector<float> rowf((size_t)RT_COUNT * nT);
+    std::vector<double> rowd((size_t)RD_COUNT * nT);
+    std::vector<float> tw((size_t)2 * nP);
+    for (int j = 0; j < nT; ++j) {
+        const double tc = (j + 0.5) * h, tt = (j + 1.0) * h;
+        const double sc = std::sin(tc), cc = std::cos(tc);
+        rowf[RT_COFPHI_C * nT + j] = (float)(dt / (radius * sc * h));
+        rowf[RT_COFPHI_T * nT + j] = (j < nT - 1) ? (float)(dt / (radius * std::sin(tt) * h)) : 0.f;
+        rowf[RT_G * nT + j] = (float)(dt * cc / (radius * sc));
+        rowf[RT_DIVS * nT + j] = (float)(std::sin(tc + 0.5 * h) / (h * sc));
+        rowf[RT_DIVN * nT + j] = (float)(std::sin(tc - 0.5 * h) / (h * sc));
+        rowf[RT_INVHS * nT + j] = (float)(1.0 / (h * sc));
+        rowd[RD_COT2H * nT + j] = cc / (2.0 * h * sc);
+        rowd[RD_INVSIN2 * nT + j] = 1.0 / (sc * sc);
+    }
+    for (int n = 0; n < nP; ++n) {
+        const double a = 2.0 * kPi * n / nP;
+        tw[2 * n] = (float)std::cos(a);
+        tw[2 * n + 1] = (float)(-std::sin(a));
+    }
+    KAM_HIP(hipSetDevice(device));
+    TabEntry e;
+    KAM_HIP(hipMalloc(&e.t.rowf, rowf.size() * sizeof(float)));
+    KAM_HIP(hipMalloc(&e.t.rowd, rowd.size() * sizeof(double)));
+    KAM_HIP(hipMalloc(&e.t.twN, tw.size() * sizeof(float)));
+    KAM_HIP(hipMemcpy(e.t.rowf, rowf.data(), rowf.size() * sizeof(float), hipMemcpyHostToDevice));
+    KAM_HIP(hipMemcpy(e.t.rowd, rowd.data(), rowd.size() * sizeof(double), hipMemcpyHostToDevice));
+    KAM_HIP(hipMemcpy(e.t.twN, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice));
+    e.refs = 1;
+    g_tabs[key] = e;
+    *out = e.t;
+    return KAM_OK;
+}
+
+void release_tables(int device, int nT, double dt, double radius) {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto it = g_tabs.find(std::make_tuple(device, nT, dt, radius));
+    if (it == g_tabs.end()) return;
+    if (--it->second.refs > 0) return;
+    (void)hipSetDevice(device);
+    (void)hipFree(it->second.t.rowf);
+    (void)hipFree(it->second.t.rowd);
+    (void)hipFree(it->second.t.twN);
+    g_tabs.erase(it);
+}
+
+void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& sd, float* mem) {
+    std::memset(&d, 0, sizeof d);
+    const double h = kPi / p.nT;
+    d.nTheta = p.nT;
+    d.nPhi = p.nP;
+    d.M = p.M;
+    d.flags = sd.geometric ? SIM_GEOMETRIC : 0;
+    d.cofTheta = (float)(sd.dt / (sd.radius * h));
+    d.invH = (float)(1.0 / h);
+    d.invH2 = 1.0 / (h * h);
+    const size_t N = (size_t)p.nT * p.nP;
+    for (int b = 0; b < 2; ++b)
+        for (int f = 0; f < 3; ++f) d.state[b][f] = mem + (size_t)(3 * b + f) * N;
+    d.spec = mem + 6 * N;
+    d.rowf = t.rowf;
+    d.rowd = t.rowd;
+    d.twN = t.twN;
+    d.nfac = p.nfac;
+    for (int k = 0; k < p.nfac; ++k) d.fac[k] = p.fac[k];
+    d.advRows = p.advRows;
+    d.fftRows = p.fftRows;
+}
+
+// ---------------------------------------------------------------------------
+// Device batches
+// ---------------------------------------------------------------------------
+DeviceBatch::~DeviceBatch() {
+    if (dSims_) (void)hipFree(dSims_);
+    if (dTiles_) (void)hipFree(dTiles_);
+}
+
+int DeviceBatch::reserve(size_t ns, size_t nt) {
+    if (ns > capSims_) {
+        if (dSims_) KAM_HIP(hipFree(dSims_));
+        dSims_ = nullptr;
+        const size_t c = ns < 16 ? 16 : ns;
+        KAM_HIP(hipMalloc(&dSims_, c * sizeof(SimDesc)));
+        capSims_ = c;
+    }
+    if (nt > capTiles_) {
+        if (dTiles_) KAM_HIP(hipFree(dTiles_));
+        dTiles_ = nullptr;
+        const size_t c = nt < 256 ? 256 : nt;
+        KAM_HIP(hipMalloc(&dTiles_, c * sizeof(Tile)));
+        capTiles_ = c;
+    }
+    return KAM_OK;
+}
+
+int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>& plans,
+                       hipStream_t st) {
+    hTiles_.clear();
+    tri_.clear();
+    advLds_ = fftLds_ = 0;
+    // advection tiles, tallest simulations first for load balance
+    advOff_ = 0;
+    for (size_t s = 0; s < sims.size(); ++s) {
+        const Plan& p = plans[s];
+        for (int j = 0; j < p.nT; j += p.advRows)
+            hTiles_.push_back(Tile{(int)s, j, std::min(p.advRows, p.nT - j), 0});
+        if (sims[s].flags & SIM_GEOMETRIC) advLds_ = std::max(advLds_, p.advLds);
+    }
+    nAdv_ = (int)hTiles_.size();
+    fftOff_ = (int)hTiles_.size();
+    for (size_t s = 0; s < sims.size(); ++s) {
+        const Plan& p = plans[s];
+        for (int j = 0; j < p.nT; j += p.fftRows)
+            hTiles_.push_back(Tile{(int)s, j, std::min(p.fftRows, p.nT - j), 0});
+        fftLds_ = std::max(fftLds_, p.fftLds);
+    }
+    nFft_ = (int)hTiles_.size() - fftOff_;
+    // tridiagonal tiles, grouped by kernel class
+    const int classes[5][2] = {{8, 8}, {16, 8}, {32, 8}, {64, 8}, {64, 16}};
+    for (auto& c : classes) {
+        const int off = (int)hTiles_.size();
+        for (size_t s = 0; s < sims.size(); ++s) {
+            const Plan& p = plans[s];
+            if (p.triL != c[0] || p.triK != c[1]) continue;
+            const int per = 256 / p.triL;
+            for (int n = 0; n < p.M; n += per)
+                hTiles_.push_back(Tile{(int)s, n, std::min(per, p.M - n), 0});
+        }
+        const int cnt = (int)hTiles_.size() - off;
+        if (cnt > 0) tri_.push_back(TriClass{c[0], c[1], off, cnt});
+    }
+    int rc = reserve(sims.size(), hTiles_.size());
+    if (rc != KAM_OK) return rc;
+    if (!sims.empty())
+        KAM_HIP(hipMemcpyAsync(dSims_, sims.data(), sims.size() * sizeof(SimDesc),
+                               hipMemcpyHostToDevice, st));
+    if (!hTiles_.empty())
+        KAM_HIP(hipMemcpyAsync(dTiles_, hTiles_.data(), hTiles_.size() * sizeof(Tile),
+                               hipMemcpyHostToDevice, st));
+    KAM_HIP(hipStreamSynchronize(st));  // host vectors may change after return
+    nsims_ = (int)sims.size();
+    return KAM_OK;
+}
+
+static const int kGridCap = 1 << 20;
+
+int DeviceBatch::launch(int phases, int parity, hipStream_t st, int* newParity,
+                        hipEvent_t* ev) {
+    if (nsims_ == 0) {
+        *newParity = parity;
+        return KAM_OK;
+    }
+    int p = parity;
+    if (phases & KAM_PHASE_ADVECT) {
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_ADVECT], st));
+        KAM_HIP(launch_advect(dSims_, dTiles_ + advOff_, nAdv_, p, advLds_, kGridCap, st));
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_ADVECT + 1], st));
+        p ^= 1;
+    }
+    if (phases & KAM_PHASE_DIVFFT) {
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT], st));
+        KAM_HIP(launch_div_fft(dSims_, dTiles_ + fftOff_, nFft_, p, fftLds_, kGridCap, st));
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT + 1], st));
+    }
+    if (phases & KAM_PHASE_TRI) {
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_TRI], st));
+        for (const TriClass& c : tri_)
+            KAM_HIP(launch_tridiag(c.L, c.K, dSims_, dTiles_ + c.off, c.count, kGridCap, st));
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_TRI + 1], st));
+    }
+    if (phases & KAM_PHASE_IFFT) {
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_IFFT], st));
+        KAM_HIP(launch_ifft_grad(dSims_, dTiles_ + fftOff_, nFft_, p, fftLds_, kGridCap, st));
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_IFFT + 1], st));
+    }
+    *newParity = p;
+    return KAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Devices
+// ---------------------------------------------------------------------------
+static std::mutex g_dev_mu;
+static std::map<int, bool> g_dev_ready;
+
+int ensure_device(int device) {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    auto it = g_dev_ready.find(device);
+    if (it != g_dev_ready.end()) {
+        KAM_HIP(hipSetDevice(device));
+        return KAM_OK;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return set_err(KAM_ENODEV, "no HIP device visible");
+    if (device < 0 || device >= n) return set_err(KAM_ENODEV, "device %d out of range (%d)", device, n);
+    hipDeviceProp_t prop;
+    KAM_HIP(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_err(KAM_ENODEV, "device %d is %s; this library is built for gfx950", device,
+                       prop.gcnArchName);
+    KAM_HIP(hipSetDevice(device));
+    KAM_HIP(configure_kernels(160 * 1024));
+    g_dev_ready[device] = true;
+    return KAM_OK;
+}
+
+}  // namespace kam
+
+// ===========================================================================
+// C ABI: single simulation
+// ===========================================================================
+using namespace kam;
+
+struct kam_sim {
+    kam_sim_desc sd;
+    Plan plan;
+    Tables tables;
+    SimDesc desc;
+    float* mem = nullptr;
+    hipStream_t stream = nullptr;
+    DeviceBatch batch;
+    int parity = 0;
+};
+
+extern "C" {
+
+const char* kam_last_error(void) { return g_err.c_str(); }
+
+const char* kam_version(void) { return "kamino-mi355x 0.1 (gfx950)"; }
+
+int kam_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    int c = 0;
+    for (int d = 0; d < n; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) == hipSuccess &&
+            std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+            ++c;
+    }
+    return c;
+}
+
+size_t kam_sim_bytes(int n_theta, int n_phi) {
+    return sim_floats(n_theta, n_phi) * sizeof(float);
+}
+
+static int check_desc(const kam_sim_desc* d, Plan& plan) {
+    if (!d) return set_err(KAM_EINVAL, "null descriptor");
+    if (!(d->dt > 0.0) || !std::isfinite(d->dt)) return set_err(KAM_EINVAL, "dt must be > 0");
+    if (!(d->radius > 0.0) || !std::isfinite(d->radius)) return set_err(KAM_EINVAL, "radius must be > 0");
+    if (d->precision != KAM_PREC_FP32) return set_err(KAM_EINVAL, "unknown precision %d", d->precision);
+    return make_plan(d->n_theta, d->n_phi, plan);
+}
+
+int kam_sim_create(const kam_sim_desc* d, kam_sim** out) {
+    if (!out) return set_err(KAM_EINVAL, "null output handle");
+    *out = nullptr;
+    Plan plan;
+    int rc = check_desc(d, plan);
+    if (rc != KAM_OK) return rc;
+    rc = ensure_device(d->device);
+    if (rc != KAM_OK) return rc;
+    kam_sim* s = new kam_sim();
+    s->sd = *d;
+    s->plan = plan;
+    rc = acquire_tables(d->device, d->n_theta, d->dt, d->radius, &s->tables);
+    if (rc != KAM_OK) {
+        delete s;
+        return rc;
+    }
+    const size_t bytes = kam_sim_bytes(d->n_theta, d->n_phi);
+    if (hipMalloc(&s->mem, bytes) != hipSuccess) {
+        release_tables(d->device, d->n_theta, d->dt, d->radius);
+        delete s;
+        return set_err(KAM_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMemsetAsync(s->mem, 0, bytes, s->stream) != hipSuccess) {
+        kam_sim_destroy(s);
+        return set_err(KAM_EHIP, "stream creation / clear failed");
+    }
+    fill_desc(s->desc, plan, s->tables, *d, s->mem);
+    std::vector<SimDesc> sims{s->desc};
+    std::vector<Plan> plans{plan};
+    rc = s->batch.build(sims, plans, s->stream);
+    if (rc != KAM_OK) {
+        kam_sim_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return KAM_OK;
+}
+
+int kam_sim_destroy(kam_sim* s) {
+    if (!s) return KAM_OK;
+    (void)hipSetDevice(s->sd.device);
+    if (s->stream) {
+        (void)hipStreamSynchronize(s->stream);
+        (void)hipStreamDestroy(s->stream);
+    }
+    if (s->mem) (void)hipFree(s->mem);
+    if (s->tables.rowf) release_tables(s->sd.device, s->sd.n_theta, s->sd.dt, s->sd.radius);
+    delete s;
+    return KAM_OK;
+}
+
+int kam_sim_set_fields(kam_sim* s, const float* u_phi, const float* u_theta, const float* rho) {
+    if (!s || !u_phi || !u_theta) return set_err(KAM_EINVAL, "null argument");
+    KAM_HIP(hipSetDevice(s->sd.device));
+    const size_t N = (size_t)s->plan.nT * s->plan.nP;
+    const size_t Nt = N - s->plan.nP;
+    KAM_HIP(hipMemcpyAsync(s->desc.state[s->parity][F_UPHI], u_phi, N * sizeof(float),
+                           hipMemcpyHostToDevice, s->stream));
+    KAM_HIP(hipMemcpyAsync(s->desc.state[s->parity][F_UTHETA], u_theta, Nt * sizeof(float),
+                           hipMemcpyHostToDevice, s->stream));
+    if (rho)
+        KAM_HIP(hipMemcpyAsync(s->desc.state[s->parity][F_RHO], rho, N * sizeof(float),
+                               hipMemcpyHostToDevice, s->stream));
+    KAM_HIP(hipStreamSynchronize(s->stream));
+    return KAM_OK;
+}
+
+int kam_sim_get_fields(kam_sim* s, float* u_phi, float* u_theta, float* rho) {
+    if (!s) return set_err(KAM_EINVAL, "null simulation");
+    KAM_HIP(hipSetDevice(s->sd.device));
+    const size_t N = (size_t)s->plan.nT * s->plan.nP;
+    const size_t Nt = N - s->plan.nP;
+    if (u_phi)
+        KAM_HIP(hipMemcpyAsync(u_phi, s->desc.state[s->parity][F_UPHI], N * sizeof(float),
+                               hipMemcpyDeviceToHost, s->stream));
+    if (u_theta)
+        KAM_HIP(hipMemcpyAsync(u_theta, s->desc.state[s->parity][F_UTHETA], Nt * sizeof(float),
+                               hipMemcpyDeviceToHost, s->stream));
+    if (rho)
+        KAM_HIP(hipMemcpyAsync(rho, s->desc.state[s->parity][F_RHO], N * sizeof(float),
+                               hipMemcpyDeviceToHost, s->stream));
+    KAM_HIP(hipStreamSynchronize(s->stream));
+    return KAM_OK;
+}
+
+int kam_sim_get_spectrum(kam_sim* s, float* spec) {
+    if (!s || !spec) return set_err(KAM_EINVAL, "null argument");
+    KAM_HIP(hipSetDevice(s->sd.device));
+    const size_t n = (size_t)s->plan.M * s->plan.nT * 2;
+    KAM_HIP(hipMemcpyAsync(spec, s->desc.spec, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    KAM_HIP(hipStreamSynchronize(s->stream));
+    return KAM_OK;
+}
+
+int kam_sim_step_phases(kam_sim* s, int phases) {
+    if (!s) return set_err(KAM_EINVAL, "null simulation");
+    if (phases & ~KAM_PHASE_ALL) return set_err(KAM_EINVAL, "unknown phase bits 0x%x", phases);
+    KAM_HIP(hipSetDevice(s->sd.device));
+    int np = s->parity;
+    int rc = s->batch.launch(phases, s->parity, s->stream, &np);
+    if (rc != KAM_OK) return rc;
+    s->parity = np;
+    return KAM_OK;
+}
+
+int kam_sim_step(kam_sim* s, int nsteps) {
+    if (!s) return set_err(KAM_EINVAL, "null simulation");
+    if (nsteps < 0) return set_err(KAM_EINVAL, "negative step count");
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = kam_sim_step_phases(s, KAM_PHASE_ALL);
+        if (rc != KAM_OK) return rc;
+    }
+    return KAM_OK;
+}
+
+int kam_sim_benchmark(kam_sim* s, int nsteps, int per_kernel, double* step_ms, double* kernel_ms,
+                      double* total_ms) {
+    if (!s || nsteps <= 0) return set_err(KAM_EINVAL, "bad benchmark arguments");
+    KAM_HIP(hipSetDevice(s->sd.device));
+    std::vector<hipEvent_t> ev((size_t)nsteps + 1);
+    hipEvent_t kev[2 * KC_COUNT];
+    for (auto& e : ev) KAM_HIP(hipEventCreate(&e));
+    for (auto& e : kev) KAM_HIP(hipEventCreate(&e));
+    double ksum[KC_COUNT] = {0, 0, 0, 0};
+    int rc = KAM_OK;
+    KAM_HIP(hipEventRecord(ev[0], s->stream));
+    for (int k = 0; k < nsteps && rc == KAM_OK; ++k) {
+        int np = s->parity;
+        rc = s->batch.launch(KAM_PHASE_ALL, s->parity, s->stream, &np, per_kernel ? kev : nullptr);
+        s->parity = np;
+        if (rc == KAM_OK && per_kernel) {
+            KAM_HIP(hipEventSynchronize(kev[2 * KC_IFFT + 1]));
+            for (int c = 0; c < KC_COUNT; ++c) {
+                float ms = 0.f;
+                KAM_HIP(hipEventElapsedTime(&ms, kev[2 * c], kev[2 * c + 1]));
+                ksum[c] += ms;
+            }
+        }
+        KAM_HIP(hipEventRecord(ev[(size_t)k + 1], s->stream));
+    }
+    KAM_HIP(hipStreamSynchronize(s->stream));
+    if (rc == KAM_OK) {
+        float tot = 0.f;
+        KAM_HIP(hipEventElapsedTime(&tot, ev[0], ev[(size_t)nsteps]));
+        if (total_ms) *total_ms = tot;
+        if (step_ms)
+            for (int k = 0; k < nsteps; ++k) {
+                float ms = 0.f;
+                KAM_HIP(hipEventElapsedTime(&ms, ev[(size_t)k], ev[(size_t)k + 1]));
+                step_ms[k] = ms;
+            }
+        if (kernel_ms)
+            for (int c = 0; c < KC_COUNT; ++c) kernel_ms[c] = per_kernel ? ksum[c] / nsteps : 0.0;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (auto& e : kev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+int kam_sim_sync(kam_sim* s) {
+    if (!s) return set_err(KAM_EINVAL, "null simulation");
+    KAM_HIP(hipSetDevice(s->sd.device));
+    KAM_HIP(hipStreamSynchronize(s->stream));
+    return KAM_OK;
+}
+
+}  // extern "C"

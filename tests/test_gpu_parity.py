@@ -1,0 +1,167 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+Every comparison starts both sides from the same FP32 state.  The bar is the
+per-step relative L2 tolerance TOL_FIELD = 1e-5 on each field, over the cells
+the oracle does not flag as branch-ambiguous (departure points within 1e-5
+cells of a discontinuity of the reference's pole samplers).
+"""
+import numpy as np
+import pytest
+
+import kaminogpu_amd as K
+from oracle import oracle as O
+from parity_util import (TOL_FIELD, div_metric, f32, f64, log, oracle_advect_geo, rel_l2,
+                         seeded_state)
+
+pytestmark = pytest.mark.gpu
+
+
+def test_device_present():
+    assert K.device_count() >= 1, K.lib().kam_last_error()
+
+
+def _gpu_phase(nt, dt, state, phases, geometric=True):
+    with K.Sim(nt, dt, geometric=geometric) as s:
+        s.set_fields(*state)
+        s.step_phases(phases)
+        return s.get_fields()
+
+
+@pytest.mark.parametrize("nt,geometric", [(32, True), (48, True), (64, True), (96, True),
+                                          (160, True), (256, True), (64, False), (256, False)])
+def test_advection_parity(nt, geometric):
+    dt = 0.5 / nt
+    state = seeded_state(nt, L=min(16, nt // 4), seed=nt, dens=1, pre_steps=1)
+    log(f"[advect] nt={nt} geometric={geometric} dt={dt}")
+    got = _gpu_phase(nt, dt, state, K.PHASE_ADVECT, geometric)
+    ref, masks, st = oracle_advect_geo(nt, dt, *state, geometric=geometric)
+    errs = [rel_l2(g, r, m) for g, r, m in zip(got, ref, masks)]
+    log(f"  rel L2 u_phi={errs[0]:.2e} u_theta={errs[1]:.2e} rho={errs[2]:.2e} "
+        f"ambiguous={[int(m.sum()) for m in masks]} escapes={st.pole_escapes}")
+    assert st.pole_escapes == 0
+    assert max(errs) < TOL_FIELD
+
+
+@pytest.mark.parametrize("nt", [32, 48, 64, 128, 160, 256])
+def test_projection_parity(nt):
+    dt = 0.5 / nt
+    state = seeded_state(nt, L=min(16, nt // 4), seed=3 * nt, dens=1)
+    (up, ut, _), _, _ = oracle_advect_geo(nt, dt, *state)  # a field with divergence
+    up, ut = f32(up, ut)
+    rho = state[2]
+    log(f"[project] nt={nt}")
+    g_up, g_ut, g_rho = _gpu_phase(nt, dt, (up, ut, rho), K.PHASE_PROJECT)
+    r_up, r_ut, _, _ = O.project(nt, 2 * nt, *f64(up, ut))
+    e = (rel_l2(g_up, r_up), rel_l2(g_ut, r_ut))
+    dm = div_metric(nt, g_up, g_ut, r_up, r_ut, up, ut)
+    d_pre = np.linalg.norm(O.divergence(nt, 2 * nt, *f64(up, ut)))
+    d_post = np.linalg.norm(O.divergence(nt, 2 * nt, r_up, r_ut))
+    log(f"  rel L2 u_phi={e[0]:.2e} u_theta={e[1]:.2e}  div: gpu-vs-oracle/pre={dm:.2e} "
+        f"oracle post/pre={d_post / d_pre:.2e}")
+    np.testing.assert_array_equal(g_rho, rho)  # the projection leaves rho alone
+    assert max(e) < TOL_FIELD
+
+
+@pytest.mark.parametrize("nt", [16, 48, 256])
+def test_spectrum_and_solve(nt):
+    """d-hat from the GPU row FFT against the oracle's shifted spectrum, and the
+    GPU per-mode solve against an FP64 dense solve of the GPU's own d-hat."""
+    nph, M = 2 * nt, nt
+    dt = 0.5 / nt
+    state = seeded_state(nt, L=min(16, nt // 4), seed=7, dens=1)
+    (up, ut, _), _, _ = oracle_advect_geo(nt, dt, *state)
+    up, ut = f32(up, ut)
+    with K.Sim(nt, dt) as s:
+        s.set_fields(up, ut, state[2])
+        s.step_phases(K.PHASE_DIVFFT)
+        dhat = s.spectrum()
+        s.step_phases(K.PHASE_TRI)
+        phat = s.spectrum()
+    _, _, spec, _ = O.project(nt, nph, *f64(up, ut))
+    ref = spec[nph // 2:nph // 2 + M, :]  # modes n = 0 .. M-1 (nIdx = n + nPhi/2)
+    nyq = spec[0, :].real                 # mode n = -M
+    got = dhat.copy()
+    got_nyq = got[0].imag.copy()
+    got[0] = got[0].real
+    e_spec = rel_l2(np.concatenate([got.real.ravel(), got.imag.ravel(), got_nyq]),
+                    np.concatenate([ref.real.ravel(), ref.imag.ravel(), nyq]))
+    # FP64 solve of the GPU's own d-hat, mode by mode
+    worst = 0.0
+    for n in range(M):
+        nn = M if n == 0 else n
+        a, b, c = O.poisson_rows(nt, nph, nn)
+        T = np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1)
+        rhs = dhat[n].astype(np.complex128)
+        if n == 0:
+            rhs = 1j * rhs.imag
+        x = np.linalg.solve(T, rhs)
+        g = phat[n].astype(np.complex128)
+        if n == 0:
+            assert np.all(g.real == 0)
+        worst = max(worst, np.linalg.norm(g - x) / max(np.linalg.norm(x), 1e-300))
+    log(f"[spectrum] nt={nt} rel L2 d-hat={e_spec:.2e} worst per-mode solve error={worst:.2e}")
+    assert e_spec < TOL_FIELD
+    assert worst < 1e-5
+
+
+CONFIGS = {
+    # config 0: 64x128, dt 0.005, L = 8, density 0.5 + 0.1 U
+    "c0": dict(nt=64, dt=0.005, L=8, dens=0),
+    # config 1: 256x512, dt 0.002, L = 16, density band
+    "c1": dict(nt=256, dt=0.002, L=16, dens=1),
+}
+
+
+@pytest.mark.parametrize("name", ["c0", "c1"])
+def test_full_step_parity(name):
+    c = CONFIGS[name]
+    nt, dt = c["nt"], c["dt"]
+    state = seeded_state(nt, L=c["L"], seed=1, dens=c["dens"], pre_steps=2, dt=dt)
+    log(f"[step] {name} nt={nt} dt={dt}")
+    got = _gpu_phase(nt, dt, state, K.PHASE_ALL)
+    _, masks, st = oracle_advect_geo(nt, dt, *state)
+    r_up, r_ut, r_rho, _ = O.step(nt, 2 * nt, dt, *f64(*state))
+    e_rho = rel_l2(got[2], r_rho, masks[2])
+    e_up = rel_l2(got[0], r_up)
+    e_ut = rel_l2(got[1], r_ut)
+    log(f"  rel L2 u_phi={e_up:.2e} u_theta={e_ut:.2e} rho={e_rho:.2e} "
+        f"ambiguous={[int(m.sum()) for m in masks]}")
+    if sum(int(m.sum()) for m in masks[:2]) == 0:
+        assert max(e_up, e_ut) < TOL_FIELD
+    assert e_rho < TOL_FIELD
+
+
+def test_multistep_drift_config0():
+    """100 steps of config 0 from the same start: long-run drift is reported,
+    bounded loosely (per-step parity is the gate)."""
+    c = CONFIGS["c0"]
+    nt, dt = c["nt"], c["dt"]
+    up, ut, rho = seeded_state(nt, L=c["L"], seed=1, dens=c["dens"])
+    with K.Sim(nt, dt) as s:
+        s.set_fields(up, ut, rho)
+        s.step(100)
+        got = s.get_fields()
+    r = f64(up, ut, rho)
+    for _ in range(100):
+        r = list(O.step(nt, 2 * nt, dt, *r)[:3])
+    errs = [rel_l2(g, x) for g, x in zip(got, r)]
+    log(f"[drift] config0 100 steps rel L2 u_phi={errs[0]:.2e} u_theta={errs[1]:.2e} rho={errs[2]:.2e}")
+    assert max(errs) < 1e-3
+
+
+def test_zero_velocity_identity_and_shift():
+    nt, dt = 64, 0.01
+    rng = np.random.default_rng(0)
+    z_up = np.zeros((nt, 2 * nt), np.float32)
+    z_ut = np.zeros((nt - 1, 2 * nt), np.float32)
+    rho = rng.uniform(0, 1, (nt, 2 * nt)).astype(np.float32)
+    g = _gpu_phase(nt, dt, (z_up, z_ut, rho), K.PHASE_ALL)
+    np.testing.assert_array_equal(g[2], rho)
+    np.testing.assert_array_equal(g[0], z_up)
+    # integer phi-shift equivariance of the whole step
+    state = seeded_state(nt, L=8, seed=5, dens=0, pre_steps=1)
+    k = 7
+    a = _gpu_phase(nt, dt, state, K.PHASE_ALL)
+    b = _gpu_phase(nt, dt, [np.roll(x, k, 1) for x in state], K.PHASE_ALL)
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(np.roll(x, k, 1), y, rtol=0, atol=2e-6 * max(1.0, np.abs(x).max()))
