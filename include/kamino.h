@@ -120,6 +120,88 @@ int kam_sim_sync(kam_sim *sim);
 /* Bytes of device memory one simulation of this shape needs. */
 size_t kam_sim_bytes(int n_theta, int n_phi);
 
+/* ==== Batching and serving engine ========================================
+ * No counterpart in the reference, which runs one simulation per process
+ * (KaminoSolver.cu:20 hard-codes device 0 and the dtor resets the device).
+ * An engine owns one stream and an HBM arena; simulations of any supported
+ * shape are admitted into slots while the arena and slot count allow, and
+ * every kam_engine_step advances all active simulations together with one
+ * batched launch per kernel (captured in a hipGraph per step parity).  A
+ * request retires after its own step count; its final fields are copied to
+ * pinned host memory on the engine stream and handed out by
+ * kam_engine_fetch. */
+
+typedef struct kam_engine kam_engine;
+
+typedef struct kam_engine_desc {
+    int device;
+    int max_slots;            /* concurrently active simulations (0 = 4096) */
+    size_t hbm_budget_bytes;  /* arena for simulation state (0 = half of free HBM) */
+    int use_graph;            /* 1: replay each step as a hipGraph */
+    int reserved;
+} kam_engine_desc;
+
+enum { KAM_REQ_ACTIVE = 1, KAM_REQ_DONE = 2, KAM_REQ_CANCELLED = 3, KAM_REQ_FAILED = 4 };
+
+typedef struct kam_req_status {
+    int state;              /* KAM_REQ_* */
+    int reserved;
+    int64_t steps_done;
+    int64_t steps_total;
+    double admit_s;         /* host clock (s since engine creation) at admission */
+    double done_s;          /* host clock when its last step was queued; 0 if not yet */
+} kam_req_status;
+
+typedef struct kam_stats {
+    int active;              /* simulations in the batch now */
+    int reserved;
+    int64_t steps;           /* engine steps since the last reset */
+    int64_t cell_steps;      /* cells advanced by those steps */
+    double gpu_ms;           /* GPU time of those steps (HIP events) */
+    double step_p50_ms;      /* per-step GPU latency percentiles over those steps */
+    double step_p99_ms;
+    double step_max_ms;
+    int64_t admitted, completed, rejected, cancelled;
+    size_t bytes_in_use, bytes_budget;
+} kam_stats;
+
+int kam_engine_create(const kam_engine_desc *desc, kam_engine **out);
+int kam_engine_destroy(kam_engine *eng);
+
+/* Admits one simulation with `steps` steps to run.  Fields are host FP32 in
+ * the layout above (rho may be NULL: zero density).  Returns KAM_EAGAIN when
+ * no slot or arena space is free now; the caller retries after steps. */
+int kam_engine_admit(kam_engine *eng, const kam_sim_desc *desc, const float *u_phi,
+                     const float *u_theta, const float *rho, int64_t steps, uint64_t *id);
+
+/* Runs nsteps shared steps of every active simulation (queued; returns once
+ * launched, except when membership changed and tables were re-uploaded). */
+int kam_engine_step(kam_engine *eng, int nsteps);
+
+int kam_engine_poll(kam_engine *eng, uint64_t id, kam_req_status *status);
+
+/* Copies a finished request's fields (waits for its copy) and forgets it. */
+int kam_engine_fetch(kam_engine *eng, uint64_t id, float *u_phi, float *u_theta, float *rho);
+
+/* Removes an active request and frees its slot. */
+int kam_engine_cancel(kam_engine *eng, uint64_t id);
+
+/* Waits for all queued engine work. */
+int kam_engine_sync(kam_engine *eng);
+
+/* Statistics since the last kam_engine_reset_stats (synchronises). */
+int kam_engine_stats(kam_engine *eng, kam_stats *stats);
+int kam_engine_reset_stats(kam_engine *eng);
+
+/* Runs nsteps steps without graphs, with HIP events around each kernel on the
+ * engine stream; kernel_ms[4] receives the mean per-step time of the
+ * advection, div+FFT, tridiagonal and inverse-FFT+gradient launches. */
+int kam_engine_profile(kam_engine *eng, int nsteps, double *kernel_ms);
+
+/* Host clock of the engine (s since creation), the time base of the request
+ * timestamps. */
+double kam_engine_now(kam_engine *eng);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,6 +30,9 @@ ABI_SYMBOLS = [
     "kam_sim_create", "kam_sim_destroy", "kam_sim_set_fields", "kam_sim_get_fields",
     "kam_sim_step", "kam_sim_step_phases", "kam_sim_get_spectrum", "kam_sim_sync",
     "kam_sim_bytes", "kam_sim_benchmark",
+    "kam_engine_create", "kam_engine_destroy", "kam_engine_admit", "kam_engine_step",
+    "kam_engine_poll", "kam_engine_fetch", "kam_engine_cancel", "kam_engine_sync",
+    "kam_engine_stats", "kam_engine_reset_stats", "kam_engine_profile", "kam_engine_now",
 ]
 
 
@@ -79,10 +82,8 @@ def lib():
 
 
 def _bind_engine(L):
-    try:
-        from . import engine as _engine  # noqa: F401
-    except ImportError:
-        return
+    from . import engine as _engine
+
     _engine.bind(L)
 
 
@@ -198,3 +199,6 @@ class KaminoSolver(Sim):
     def stepForward(self, time_step=None):  # noqa: N802 -- reference name
         self.step(1)
         self.time_elapsed += self.desc.dt if time_step is None else time_step
+
+
+from .engine import Engine  # noqa: E402,F401

@@ -1,0 +1,525 @@
+// kamino_engine.hip -- batching and serving engine (kam_engine_*).
+//
+// One engine = one device, one stream, one HBM arena.  Active simulations of
+// any supported shape share every step: the device batch holds their
+// descriptors and tile lists, and a step is the four batched launches of
+// DeviceBatch::launch, replayed from a hipGraph captured per step parity.
+// Membership changes (admission, retirement, cancellation) rebuild the tables
+// and drop the graphs.  All device work is ordered on the engine stream, so a
+// retired simulation's arena block can be handed to the next admission at
+// once: the new upload is queued after the old download.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <unordered_map>
+#include <vector>
+
+#include "kamino_runtime.h"
+
+namespace kam {
+
+// First-fit allocator over one device allocation.
+class Arena {
+public:
+    int init(size_t bytes) {
+        bytes = bytes & ~(size_t)255;
+        KAM_HIP(hipMalloc(&base_, bytes));
+        size_ = bytes;
+        free_[0] = bytes;
+        return KAM_OK;
+    }
+    ~Arena() {
+        if (base_) (void)hipFree(base_);
+    }
+    bool alloc(size_t bytes, size_t* off) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                *off = it->first;
+                const size_t rest = it->second - bytes, o = it->first + bytes;
+                free_.erase(it);
+                if (rest) free_[o] = rest;
+                used_ += bytes;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t bytes) {
+        bytes = (bytes + 255) & ~(size_t)255;
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    char* base() const { return base_; }
+    size_t size() const { return size_; }
+    size_t used() const { return used_; }
+
+private:
+    char* base_ = nullptr;
+    size_t size_ = 0, used_ = 0;
+    std::map<size_t, size_t> free_;
+};
+
+// Pinned host buffers recycled by size.
+class PinnedPool {
+public:
+    ~PinnedPool() {
+        for (auto& kv : free_)
+            for (void* p : kv.second) (void)hipHostFree(p);
+    }
+    void* get(size_t bytes) {
+        auto& v = free_[bytes];
+        if (!v.empty()) {
+            void* p = v.back();
+            v.pop_back();
+            return p;
+        }
+        void* p = nullptr;
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+        return p;
+    }
+    void put(void* p, size_t bytes) { free_[bytes].push_back(p); }
+
+private:
+    std::map<size_t, std::vector<void*>> free_;
+};
+
+struct Request {
+    uint64_t id = 0;
+    kam_sim_desc sd{};
+    Plan plan;
+    Tables tables;
+    SimDesc desc{};
+    size_t off = 0, bytes = 0;
+    int64_t total = 0, done = 0;
+    int state = KAM_REQ_ACTIVE;
+    double admit_s = 0, done_s = 0;
+    void* host = nullptr;  // pinned copy of the final fields
+    size_t hostBytes = 0;
+    hipEvent_t copied = nullptr;
+};
+
+}  // namespace kam
+
+using namespace kam;
+
+struct kam_engine {
+    kam_engine_desc ed{};
+    hipStream_t stream = nullptr;
+    Arena arena;
+    PinnedPool pinned;
+    DeviceBatch batch;
+    std::unordered_map<uint64_t, Request> reqs;
+    std::vector<uint64_t> active;  // batch order
+    uint64_t nextId = 1;
+    int parity = 0;
+    bool dirty = true;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // statistics
+    std::deque<hipEvent_t> stepEvents;  // recorded after each step, not yet collected
+    std::vector<hipEvent_t> eventPool;
+    hipEvent_t startEvent = nullptr;    // marks the beginning of the stats window
+    bool windowOpen = false;
+    std::vector<float> stepMs;
+    int64_t steps = 0, cellSteps = 0, curCells = 0;
+    int64_t admitted = 0, completed = 0, rejected = 0, cancelled = 0;
+    std::chrono::steady_clock::time_point t0;
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+int drop_graphs(kam_engine* e) {
+    for (auto& g : e->graph)
+        if (g) {
+            KAM_HIP(hipGraphExecDestroy(g));
+            g = nullptr;
+        }
+    return KAM_OK;
+}
+
+int rebuild(kam_engine* e) {
+    std::vector<SimDesc> sims;
+    std::vector<Plan> plans;
+    sims.reserve(e->active.size());
+    plans.reserve(e->active.size());
+    // tallest simulations first: their tiles start first
+    std::stable_sort(e->active.begin(), e->active.end(), [e](uint64_t a, uint64_t b) {
+        return e->reqs[a].plan.nT > e->reqs[b].plan.nT;
+    });
+    e->curCells = 0;
+    for (uint64_t id : e->active) {
+        const Request& r = e->reqs[id];
+        sims.push_back(r.desc);
+        plans.push_back(r.plan);
+        e->curCells += (int64_t)r.plan.nT * r.plan.nP;
+    }
+    int rc = drop_graphs(e);
+    if (rc != KAM_OK) return rc;
+    rc = e->batch.build(sims, plans, e->stream);
+    if (rc != KAM_OK) return rc;
+    e->dirty = false;
+    return KAM_OK;
+}
+
+hipEvent_t take_event(kam_engine* e) {
+    if (!e->eventPool.empty()) {
+        hipEvent_t ev = e->eventPool.back();
+        e->eventPool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+}
+
+// Turns completed step events into per-step durations.  With `wait`, blocks
+// on the stream first so every recorded step is collected.
+int collect(kam_engine* e, bool wait) {
+    if (wait) KAM_HIP(hipStreamSynchronize(e->stream));
+    while (!e->stepEvents.empty()) {
+        hipEvent_t ev = e->stepEvents.front();
+        if (!wait && hipEventQuery(ev) != hipSuccess) break;
+        if (e->windowOpen) {
+            float ms = 0.f;
+            KAM_HIP(hipEventElapsedTime(&ms, e->startEvent, ev));
+            e->stepMs.push_back(ms);
+        }
+        // the event becomes the start of the next interval
+        if (e->startEvent) e->eventPool.push_back(e->startEvent);
+        e->startEvent = ev;
+        e->windowOpen = true;
+        e->stepEvents.pop_front();
+    }
+    return KAM_OK;
+}
+
+int retire(kam_engine* e, Request& r, int state) {
+    if (state == KAM_REQ_DONE) {
+        const size_t N = (size_t)r.plan.nT * r.plan.nP;
+        r.hostBytes = 3 * N * sizeof(float);
+        r.host = e->pinned.get(r.hostBytes);
+        if (!r.host) return set_err(KAM_ENOMEM, "pinned host allocation failed");
+        float* h = static_cast<float*>(r.host);
+        for (int f = 0; f < 3; ++f)
+            KAM_HIP(hipMemcpyAsync(h + f * N, r.desc.state[e->parity][f], N * sizeof(float),
+                                   hipMemcpyDeviceToHost, e->stream));
+        KAM_HIP(hipEventCreateWithFlags(&r.copied, hipEventDisableTiming));
+        KAM_HIP(hipEventRecord(r.copied, e->stream));
+        r.done_s = e->now();
+    }
+    r.state = state;
+    e->arena.release(r.off, r.bytes);
+    release_tables(e->ed.device, r.sd.n_theta, r.sd.dt, r.sd.radius);
+    e->active.erase(std::find(e->active.begin(), e->active.end(), r.id));
+    e->dirty = true;
+    return KAM_OK;
+}
+
+int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
+    if (e->dirty) {
+        int rc = rebuild(e);
+        if (rc != KAM_OK) return rc;
+    }
+    if (e->active.empty()) return KAM_OK;
+    int np = e->parity;
+    if (e->ed.use_graph && !kernelEvents) {
+        hipGraphExec_t& g = e->graph[e->parity];
+        if (!g) {
+            hipGraph_t graph = nullptr;
+            KAM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+            int rc = e->batch.launch(KAM_PHASE_ALL, e->parity, e->stream, &np);
+            hipError_t ce = hipStreamEndCapture(e->stream, &graph);
+            if (rc != KAM_OK) return rc;
+            if (ce != hipSuccess) return set_err(KAM_EHIP, "graph capture: %s", hipGetErrorString(ce));
+            ce = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ce != hipSuccess) return set_err(KAM_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
+        }
+        KAM_HIP(hipGraphLaunch(g, e->stream));
+        np = e->parity ^ 1;
+    } else {
+        int rc = e->batch.launch(KAM_PHASE_ALL, e->parity, e->stream, &np, kernelEvents);
+        if (rc != KAM_OK) return rc;
+    }
+    e->parity = np;
+    hipEvent_t ev = take_event(e);
+    if (!ev) return set_err(KAM_EHIP, "event creation failed");
+    KAM_HIP(hipEventRecord(ev, e->stream));
+    e->stepEvents.push_back(ev);
+    e->steps++;
+    e->cellSteps += e->curCells;
+    // advance every request; retire the finished ones
+    std::vector<uint64_t> finished;
+    for (uint64_t id : e->active) {
+        Request& r = e->reqs[id];
+        if (++r.done >= r.total) finished.push_back(id);
+    }
+    for (uint64_t id : finished) {
+        int rc = retire(e, e->reqs[id], KAM_REQ_DONE);
+        if (rc != KAM_OK) return rc;
+        e->completed++;
+    }
+    if (e->stepEvents.size() > 1024) return collect(e, false);
+    return KAM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kam_engine_create(const kam_engine_desc* d, kam_engine** out) {
+    if (!d || !out) return set_err(KAM_EINVAL, "null argument");
+    *out = nullptr;
+    int rc = ensure_device(d->device);
+    if (rc != KAM_OK) return rc;
+    kam_engine* e = new kam_engine();
+    e->ed = *d;
+    if (e->ed.max_slots <= 0) e->ed.max_slots = 4096;
+    e->t0 = std::chrono::steady_clock::now();
+    size_t budget = d->hbm_budget_bytes;
+    if (budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            delete e;
+            return set_err(KAM_EHIP, "hipMemGetInfo failed");
+        }
+        budget = fr / 2;
+    }
+    if ((rc = e->arena.init(budget)) != KAM_OK ||
+        hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete e;
+        return rc != KAM_OK ? rc : set_err(KAM_EHIP, "stream creation failed");
+    }
+    *out = e;
+    return KAM_OK;
+}
+
+int kam_engine_destroy(kam_engine* e) {
+    if (!e) return KAM_OK;
+    (void)hipSetDevice(e->ed.device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graphs(e);
+    for (auto& kv : e->reqs) {
+        Request& r = kv.second;
+        if (r.state == KAM_REQ_ACTIVE) release_tables(e->ed.device, r.sd.n_theta, r.sd.dt, r.sd.radius);
+        if (r.host) e->pinned.put(r.host, r.hostBytes);
+        if (r.copied) (void)hipEventDestroy(r.copied);
+    }
+    for (hipEvent_t ev : e->stepEvents) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : e->eventPool) (void)hipEventDestroy(ev);
+    if (e->startEvent) (void)hipEventDestroy(e->startEvent);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    return KAM_OK;
+}
+
+int kam_engine_admit(kam_engine* e, const kam_sim_desc* d, const float* u_phi,
+                     const float* u_theta, const float* rho, int64_t steps, uint64_t* id) {
+    if (!e || !d || !u_phi || !u_theta || !id) return set_err(KAM_EINVAL, "null argument");
+    if (steps <= 0) return set_err(KAM_EINVAL, "steps must be > 0");
+    if (d->device != e->ed.device) return set_err(KAM_EINVAL, "simulation device differs from the engine's");
+    if (!(d->dt > 0.0) || !(d->radius > 0.0) || d->precision != KAM_PREC_FP32)
+        return set_err(KAM_EINVAL, "bad dt / radius / precision");
+    KAM_HIP(hipSetDevice(e->ed.device));
+    Request r;
+    int rc = make_plan(d->n_theta, d->n_phi, r.plan);
+    if (rc != KAM_OK) return rc;
+    if ((int)e->active.size() >= e->ed.max_slots) {
+        e->rejected++;
+        return set_err(KAM_EAGAIN, "all %d slots busy", e->ed.max_slots);
+    }
+    r.bytes = kam_sim_bytes(d->n_theta, d->n_phi);
+    if (!e->arena.alloc(r.bytes, &r.off)) {
+        e->rejected++;
+        return set_err(KAM_EAGAIN, "HBM budget exhausted (%zu of %zu bytes in use)", e->arena.used(),
+                       e->arena.size());
+    }
+    rc = acquire_tables(e->ed.device, d->n_theta, d->dt, d->radius, &r.tables);
+    if (rc != KAM_OK) {
+        e->arena.release(r.off, r.bytes);
+        return rc;
+    }
+    r.id = e->nextId++;
+    r.sd = *d;
+    r.total = steps;
+    r.admit_s = e->now();
+    float* mem = reinterpret_cast<float*>(e->arena.base() + r.off);
+    fill_desc(r.desc, r.plan, r.tables, *d, mem);
+    const size_t N = (size_t)d->n_theta * d->n_phi;
+    // both state copies' u_theta padding row and rho default to zero
+    KAM_HIP(hipMemsetAsync(mem, 0, 6 * N * sizeof(float), e->stream));
+    KAM_HIP(hipMemcpyAsync(r.desc.state[e->parity][F_UPHI], u_phi, N * sizeof(float),
+                           hipMemcpyHostToDevice, e->stream));
+    KAM_HIP(hipMemcpyAsync(r.desc.state[e->parity][F_UTHETA], u_theta, (N - d->n_phi) * sizeof(float),
+                           hipMemcpyHostToDevice, e->stream));
+    if (rho)
+        KAM_HIP(hipMemcpyAsync(r.desc.state[e->parity][F_RHO], rho, N * sizeof(float),
+                               hipMemcpyHostToDevice, e->stream));
+    *id = r.id;
+    e->active.push_back(r.id);
+    e->reqs[r.id] = r;
+    e->admitted++;
+    e->dirty = true;
+    return KAM_OK;
+}
+
+int kam_engine_step(kam_engine* e, int nsteps) {
+    if (!e || nsteps < 0) return set_err(KAM_EINVAL, "bad arguments");
+    KAM_HIP(hipSetDevice(e->ed.device));
+    for (int k = 0; k < nsteps; ++k) {
+        int rc = one_step(e, nullptr);
+        if (rc != KAM_OK) return rc;
+    }
+    return KAM_OK;
+}
+
+int kam_engine_profile(kam_engine* e, int nsteps, double* kernel_ms) {
+    if (!e || nsteps <= 0) return set_err(KAM_EINVAL, "bad arguments");
+    KAM_HIP(hipSetDevice(e->ed.device));
+    hipEvent_t kev[2 * KC_COUNT];
+    for (auto& ev : kev) KAM_HIP(hipEventCreate(&ev));
+    double sum[KC_COUNT] = {0, 0, 0, 0};
+    int rc = KAM_OK;
+    for (int k = 0; k < nsteps && rc == KAM_OK; ++k) {
+        rc = one_step(e, kev);
+        if (rc != KAM_OK) break;
+        KAM_HIP(hipStreamSynchronize(e->stream));
+        for (int c = 0; c < KC_COUNT; ++c) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, kev[2 * c], kev[2 * c + 1]) == hipSuccess) sum[c] += ms;
+        }
+    }
+    for (auto& ev : kev) (void)hipEventDestroy(ev);
+    if (rc == KAM_OK && kernel_ms)
+        for (int c = 0; c < KC_COUNT; ++c) kernel_ms[c] = sum[c] / nsteps;
+    return rc;
+}
+
+int kam_engine_poll(kam_engine* e, uint64_t id, kam_req_status* st) {
+    if (!e || !st) return set_err(KAM_EINVAL, "null argument");
+    auto it = e->reqs.find(id);
+    if (it == e->reqs.end()) return set_err(KAM_ENOTFOUND, "unknown request %llu", (unsigned long long)id);
+    const Request& r = it->second;
+    st->state = r.state;
+    if (r.state == KAM_REQ_DONE && r.copied && hipEventQuery(r.copied) != hipSuccess)
+        st->state = KAM_REQ_ACTIVE;  // the final copy is still in flight
+    st->reserved = 0;
+    st->steps_done = r.done;
+    st->steps_total = r.total;
+    st->admit_s = r.admit_s;
+    st->done_s = r.done_s;
+    return KAM_OK;
+}
+
+int kam_engine_fetch(kam_engine* e, uint64_t id, float* u_phi, float* u_theta, float* rho) {
+    if (!e) return set_err(KAM_EINVAL, "null engine");
+    auto it = e->reqs.find(id);
+    if (it == e->reqs.end()) return set_err(KAM_ENOTFOUND, "unknown request %llu", (unsigned long long)id);
+    Request& r = it->second;
+    if (r.state != KAM_REQ_DONE) return set_err(KAM_ESTATE, "request %llu has not finished",
+                                                 (unsigned long long)id);
+    KAM_HIP(hipEventSynchronize(r.copied));
+    const size_t N = (size_t)r.plan.nT * r.plan.nP;
+    const float* h = static_cast<const float*>(r.host);
+    if (u_phi) std::memcpy(u_phi, h, N * sizeof(float));
+    if (u_theta) std::memcpy(u_theta, h + N, (N - r.plan.nP) * sizeof(float));
+    if (rho) std::memcpy(rho, h + 2 * N, N * sizeof(float));
+    e->pinned.put(r.host, r.hostBytes);
+    (void)hipEventDestroy(r.copied);
+    e->reqs.erase(it);
+    return KAM_OK;
+}
+
+int kam_engine_cancel(kam_engine* e, uint64_t id) {
+    if (!e) return set_err(KAM_EINVAL, "null engine");
+    auto it = e->reqs.find(id);
+    if (it == e->reqs.end()) return set_err(KAM_ENOTFOUND, "unknown request %llu", (unsigned long long)id);
+    if (it->second.state != KAM_REQ_ACTIVE) {
+        e->reqs.erase(it);
+        return KAM_OK;
+    }
+    int rc = retire(e, it->second, KAM_REQ_CANCELLED);
+    e->cancelled++;
+    e->reqs.erase(id);
+    return rc;
+}
+
+int kam_engine_sync(kam_engine* e) {
+    if (!e) return set_err(KAM_EINVAL, "null engine");
+    KAM_HIP(hipSetDevice(e->ed.device));
+    return collect(e, true);
+}
+
+int kam_engine_stats(kam_engine* e, kam_stats* s) {
+    if (!e || !s) return set_err(KAM_EINVAL, "null argument");
+    int rc = kam_engine_sync(e);
+    if (rc != KAM_OK) return rc;
+    std::memset(s, 0, sizeof *s);
+    s->active = (int)e->active.size();
+    s->steps = e->steps;
+    s->cell_steps = e->cellSteps;
+    std::vector<float> v(e->stepMs);
+    double sum = 0;
+    for (float x : v) sum += x;
+    s->gpu_ms = sum;
+    if (!v.empty()) {
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) {
+            const double pos = p * (v.size() - 1);
+            const size_t lo = (size_t)std::floor(pos), hi = std::min(lo + 1, v.size() - 1);
+            return (double)v[lo] + (pos - lo) * ((double)v[hi] - v[lo]);
+        };
+        s->step_p50_ms = pct(0.50);
+        s->step_p99_ms = pct(0.99);
+        s->step_max_ms = v.back();
+    }
+    s->admitted = e->admitted;
+    s->completed = e->completed;
+    s->rejected = e->rejected;
+    s->cancelled = e->cancelled;
+    s->bytes_in_use = e->arena.used();
+    s->bytes_budget = e->arena.size();
+    return KAM_OK;
+}
+
+int kam_engine_reset_stats(kam_engine* e) {
+    if (!e) return set_err(KAM_EINVAL, "null engine");
+    KAM_HIP(hipSetDevice(e->ed.device));
+    int rc = collect(e, true);
+    if (rc != KAM_OK) return rc;
+    // the next window starts at an event recorded now
+    if (!e->startEvent) {
+        e->startEvent = take_event(e);
+        if (!e->startEvent) return set_err(KAM_EHIP, "event creation failed");
+    }
+    KAM_HIP(hipEventRecord(e->startEvent, e->stream));
+    e->windowOpen = true;
+    e->stepMs.clear();
+    e->steps = 0;
+    e->cellSteps = 0;
+    e->admitted = e->completed = e->rejected = e->cancelled = 0;
+    return KAM_OK;
+}
+
+double kam_engine_now(kam_engine* e) { return e ? e->now() : 0.0; }
+
+}  // extern "C"

@@ -1,0 +1,50 @@
+"""CPU checks of the C ABI boundary: the library loads without a GPU, exports
+every function include/kamino.h declares, and rejects bad arguments with
+status codes (never by exiting)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+import kaminogpu_amd as K
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "kamino.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kam_[a-z_]+)\s*\(", text)))
+
+
+def test_header_declares_the_python_symbol_list():
+    assert declared_functions() == sorted(K.ABI_SYMBOLS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = C.CDLL(K.LIB_PATH)
+    missing = [s for s in declared_functions() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_bad_shapes_are_rejected_without_a_gpu():
+    L = K.lib()
+    h = C.c_void_p()
+    for nt, nph, dt in [(3, 6, 0.1), (64, 100, 0.1), (64, 128, 0.0), (7 * 11, 154, 0.1), (2048, 4096, 0.1)]:
+        d = K.SimDesc(nt, nph, 1.0, dt, 1, 0, 0)
+        rc = L.kam_sim_create(C.byref(d), C.byref(h))
+        assert rc != K.KAM_OK
+        assert L.kam_last_error().decode()
+
+
+def test_sim_bytes_is_28_per_cell():
+    assert K.lib().kam_sim_bytes(256, 512) == 28 * 256 * 512
+
+
+def test_no_device_in_this_container_is_reported_not_fatal():
+    n = K.device_count()
+    assert n >= 0
+    if n == 0:
+        with pytest.raises(K.KaminoError):
+            K.Sim(64, 0.005)

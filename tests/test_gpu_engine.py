@@ -1,0 +1,150 @@
+"""GPU tests of batching and serving: every simulation's result is the same
+alone (kam_sim, plain launches) and inside any batch or serving mix (engine,
+graph-replayed batched launches).  A simulation's tiles do not depend on what
+else is in the batch, so the equality is bitwise."""
+import numpy as np
+import pytest
+
+import kaminogpu_amd as K
+from oracle import oracle as O
+from parity_util import TOL_FIELD, f64, log, oracle_advect_geo, rel_l2, seeded_state
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [32, 48, 64, 96, 128, 160, 192, 256]
+
+
+def alone(nt, dt, state, steps, geometric=True):
+    with K.Sim(nt, dt, geometric=geometric) as s:
+        s.set_fields(*state)
+        s.step(steps)
+        return s.get_fields()
+
+
+def make_request(nt, seed):
+    return dict(nt=nt, dt=0.5 / nt, state=seeded_state(nt, L=min(16, nt // 4), seed=seed, dens=1))
+
+
+def test_uneven_batch_equals_alone_bitwise():
+    reqs = [make_request(nt, 100 + k) for k, nt in enumerate(SIZES)]
+    with K.Engine() as eng:
+        ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=6) for r in reqs]
+        assert all(i is not None for i in ids)
+        eng.step(6)
+        eng.sync()
+        for r, i in zip(reqs, ids):
+            assert eng.poll(i).state == K.engine.REQ_DONE
+            got = eng.fetch(i, r["nt"])
+            ref = alone(r["nt"], r["dt"], r["state"], 6)
+            for g, x in zip(got, ref):
+                np.testing.assert_array_equal(g, x)
+    log(f"[engine] uneven batch of {len(reqs)} sims bitwise equal to alone")
+
+
+def test_graph_and_direct_launches_agree():
+    reqs = [make_request(nt, 7 + k) for k, nt in enumerate([64, 96, 64])]
+    out = []
+    for use_graph in (True, False):
+        with K.Engine(use_graph=use_graph) as eng:
+            ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=5) for r in reqs]
+            eng.step(5)
+            out.append([eng.fetch(i, r["nt"]) for r, i in zip(reqs, ids)])
+    for a, b in zip(*out):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+
+
+def test_one_batched_step_matches_oracle():
+    reqs = [make_request(nt, 40 + k) for k, nt in enumerate([48, 64, 128])]
+    with K.Engine() as eng:
+        ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=1) for r in reqs]
+        eng.step(1)
+        for r, i in zip(reqs, ids):
+            nt, dt = r["nt"], r["dt"]
+            got = eng.fetch(i, nt)
+            ref = O.step(nt, 2 * nt, dt, *f64(*r["state"]))
+            _, masks, _ = oracle_advect_geo(nt, dt, *r["state"])
+            errs = [rel_l2(got[0], ref[0]), rel_l2(got[1], ref[1]), rel_l2(got[2], ref[2], masks[2])]
+            log(f"[engine] nt={nt} one step rel L2 {['%.2e' % e for e in errs]}")
+            assert max(errs) < TOL_FIELD
+
+
+def test_serving_mix_continuous_admission():
+    """Requests arrive while others run; each is admitted as a slot frees up,
+    runs its own step count inside shared steps, and equals its solo run."""
+    rng = np.random.default_rng(5)
+    pending = []
+    for k in range(10):
+        nt = int(rng.choice([32, 48, 64, 96]))
+        r = make_request(nt, 300 + k)
+        r["steps"] = int(rng.integers(2, 9))
+        r["arrive"] = int(rng.integers(0, 12))
+        pending.append(r)
+    pending.sort(key=lambda r: r["arrive"])
+    done = {}
+    with K.Engine(max_slots=3) as eng:
+        running = {}
+        t = 0
+        while pending or running:
+            while pending and pending[0]["arrive"] <= t:
+                rid = eng.admit(pending[0]["nt"], pending[0]["dt"], *pending[0]["state"],
+                                steps=pending[0]["steps"])
+                if rid is None:
+                    break  # no slot now; retry after the next step
+                running[rid] = pending.pop(0)
+            eng.step(1)
+            t += 1
+            for rid in list(running):
+                st = eng.poll(rid)
+                if st.state == K.engine.REQ_DONE:
+                    r = running.pop(rid)
+                    done[id(r)] = (r, eng.fetch(rid, r["nt"]))
+            assert t < 500
+        s = eng.stats()
+        assert s.completed == 10 and s.active == 0
+    for r, got in done.values():
+        ref = alone(r["nt"], r["dt"], r["state"], r["steps"])
+        for g, x in zip(got, ref):
+            np.testing.assert_array_equal(g, x)
+    log(f"[serve] 10 requests through 3 slots in {t} shared steps, all equal to solo runs")
+
+
+def test_hbm_budget_admission_and_cancel():
+    nt = 64
+    r = make_request(nt, 1)
+    per = K.lib().kam_sim_bytes(nt, 2 * nt)
+    with K.Engine(hbm_budget_bytes=2 * per + 4096) as eng:
+        a = eng.admit(nt, r["dt"], *r["state"], steps=100)
+        b = eng.admit(nt, r["dt"], *r["state"], steps=100)
+        assert a is not None and b is not None
+        assert eng.admit(nt, r["dt"], *r["state"], steps=100) is None  # budget exhausted
+        eng.step(2)
+        eng.cancel(a)
+        c = eng.admit(nt, r["dt"], *r["state"], steps=3)
+        assert c is not None
+        eng.step(3)
+        eng.sync()  # poll reports ACTIVE until the final copy to the host has landed
+        assert eng.poll(c).state == K.engine.REQ_DONE
+        got = eng.fetch(c, nt)
+        ref = alone(nt, r["dt"], r["state"], 3)
+        for g, x in zip(got, ref):
+            np.testing.assert_array_equal(g, x)
+        s = eng.stats()
+        assert s.rejected == 1 and s.cancelled == 1 and s.active == 1
+
+
+def test_engine_stats_latency():
+    reqs = [make_request(256, 9 + k) for k in range(4)]
+    with K.Engine() as eng:
+        for r in reqs:
+            eng.admit(256, r["dt"], *r["state"], steps=1000)
+        eng.step(5)
+        eng.reset_stats()
+        eng.step(50)
+        s = eng.stats()
+        assert s.steps == 50 and s.cell_steps == 50 * 4 * 256 * 512
+        assert 0 < s.step_p50_ms <= s.step_p99_ms <= s.step_max_ms
+        k = eng.profile(3)
+        assert (k > 0).all()
+        log(f"[engine] 4x256x512: p50 {s.step_p50_ms:.3f} ms p99 {s.step_p99_ms:.3f} ms "
+            f"kernels {np.round(k, 4)}")

@@ -109,10 +109,12 @@ CONFIGS = {
     "c0": dict(nt=64, dt=0.005, L=8, dens=0),
     # config 1: 256x512, dt 0.002, L = 16, density band
     "c1": dict(nt=256, dt=0.002, L=16, dens=1),
+    # config 2: 1024x2048, dt 0.5/nTheta (unspecified there), L = 32, density band
+    "c2": dict(nt=1024, dt=0.5 / 1024, L=32, dens=1),
 }
 
 
-@pytest.mark.parametrize("name", ["c0", "c1"])
+@pytest.mark.parametrize("name", ["c0", "c1", "c2"])
 def test_full_step_parity(name):
     c = CONFIGS[name]
     nt, dt = c["nt"], c["dt"]
