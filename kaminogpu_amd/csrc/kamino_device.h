@@ -53,16 +53,20 @@ struct SimDesc {
     int fac[12];
     int advRows;           // rows per advection tile
     int fftRows;           // rows per FFT tile
-    int pad0, pad1;
+    unsigned magicP;       // ceil(2^32 / nPhi): i / nPhi == umulhi(i, magicP) for i < 2^16
+    float invNP;           // 1 / nPhi
+    int advCols;           // columns per advection tile (W <= min(nPhi, 255))
+    unsigned magicW[3];    // ceil(2^32 / (W + k)), k = 0, 1, 2
 };
 
 // One unit of work: rows [row0, row0 + rows) of simulation `sim` (or, for the
-// tridiagonal kernel, modes [row0, row0 + rows)).
+// tridiagonal kernel, modes [row0, row0 + rows)); advection tiles also start
+// at column col0 and span the simulation's advCols columns.
 struct Tile {
     int sim;
     int row0;
     int rows;
-    int pad;
+    int col0;
 };
 
 }  // namespace kam

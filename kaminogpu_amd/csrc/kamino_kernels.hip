@@ -24,71 +24,83 @@ enum Kind { K_VPHI = 0, K_VTHETA = 1, K_CENTERED = 2 };
 // over the poles (validateCoord, :11-29) and picks one of two bilinear
 // branches; this is the same map in index space.  `bj + fth` is the
 // offset-shifted theta coordinate in cells, `bi + fphi` the phi coordinate.
+// Fields are read through buffer resources (32-bit offsets, hardware range
+// check), and the branch choice is made with selects, not jumps.
 // ---------------------------------------------------------------------------
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+
+__device__ __forceinline__ Rsrc make_rsrc(const float* p, int nfloats) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, nfloats * 4, 0x00020000);
+}
+__device__ __forceinline__ float ldf(Rsrc r, int elem) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, elem * 4, 0, 0));
+}
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
 
-__device__ __forceinline__ int wrap_phi(int i, int nPhi) {
-    if ((unsigned)i >= (unsigned)nPhi) {
-        i %= nPhi;
-        if (i < 0) i += nPhi;
-    }
-    return i;
-}
+// Per-shape constants a sampler needs.
+struct Grid {
+    int nT, nP, half;
+    float invNP;
+};
 
 template <int KIND>
-__device__ __forceinline__ float sample(const float* __restrict__ f, int nTheta, int nPhi, int bi,
-                                        float fphi, int bj, float fth) {
-    const int rows = (KIND == K_VTHETA) ? nTheta - 1 : nTheta;
-    const int poleRow = (KIND == K_VTHETA) ? nTheta - 2 : nTheta - 1;
+__device__ __forceinline__ float sample(Rsrc f, const Grid& g, int bi, float fphi, int bj, float fth) {
+    const int rows = (KIND == K_VTHETA) ? g.nT - 1 : g.nT;
+    const int poleRow = (KIND == K_VTHETA) ? g.nT - 2 : g.nT - 1;
     const float kp = floorf(fphi);
     int ip = bi + (int)kp;
     const float ap = fphi - kp;
     const float kt = floorf(fth);
     int jt = bj + (int)kt;
     float at = fth - kt;
-    // fold: t < 0 -> -t, and t > nTheta -> 2 nTheta - t, each adding half a turn in phi
-    bool flip = false;
-    if (jt < 0) {
-        flip = true;
-        if (at > 0.f) {
-            jt = -jt - 1;
-            at = 1.f - at;
-        } else {
-            jt = -jt;
-        }
-    } else if (jt > nTheta || (jt == nTheta && at > 0.f)) {
-        flip = true;
-        if (at > 0.f) {
-            jt = 2 * nTheta - jt - 1;
-            at = 1.f - at;
-        } else {
-            jt = 2 * nTheta - jt;
-        }
-    }
-    const int half = nPhi >> 1;
-    if (flip) ip += half;
-    ip = wrap_phi(ip, nPhi);
-    int ip1 = ip + 1;
-    if (ip1 == nPhi) ip1 = 0;
-    if ((jt == 0 && flip) || jt == poleRow) {
-        // pole branch: the same row at phi and phi + pi
-        const int r = min(max(jt, 0), rows - 1);
-        const float* row = f + (size_t)r * nPhi;
-        int ia = ip + half;
-        if (ia >= nPhi) ia -= nPhi;
+    float a0, a1, b0, b1, w;
+    if (jt >= 1 && jt < poleRow) {
+        // ordinary bilinear: no fold, no pole branch, both rows in range
+        ip -= __mul24(g.nP, (int)floorf((float)ip * g.invNP));
+        ip += (ip < 0) ? g.nP : 0;
+        ip -= (ip >= g.nP) ? g.nP : 0;
+        int ip1 = ip + 1;
+        ip1 = (ip1 == g.nP) ? 0 : ip1;
+        const int oA = __mul24(jt, g.nP) + ip, oA1 = __mul24(jt, g.nP) + ip1;
+        a0 = ldf(f, oA);
+        a1 = ldf(f, oA1);
+        b0 = ldf(f, oA + g.nP);
+        b1 = ldf(f, oA1 + g.nP);
+        w = at;
+    } else {
+        // fold: t < 0 -> -t and t > nTheta -> 2 nTheta - t, each adding half a turn in phi
+        const bool frac = at > 0.f;
+        const bool neg = jt < 0;
+        const bool high = (jt > g.nT) | ((jt == g.nT) & frac);
+        const bool flip = neg | high;
+        const int refl = (neg ? -jt : 2 * g.nT - jt) - (int)frac;
+        jt = flip ? refl : jt;
+        at = (flip & frac) ? 1.f - at : at;
+        ip += flip ? g.half : 0;
+        // ip mod nPhi (the float quotient is off by at most one)
+        ip -= __mul24(g.nP, (int)floorf((float)ip * g.invNP));
+        ip += (ip < 0) ? g.nP : 0;
+        ip -= (ip >= g.nP) ? g.nP : 0;
+        int ip1 = ip + 1;
+        ip1 = (ip1 == g.nP) ? 0 : ip1;
+        // pole branch: the same row at phi and at phi + pi (alpha halved except for u_phi)
+        const bool pole = ((jt == 0) & flip) | (jt == poleRow);
+        int ia = ip + g.half;
+        ia -= (ia >= g.nP) ? g.nP : 0;
         int ib = ia + 1;
-        if (ib == nPhi) ib = 0;
-        const float lo = lerpf(row[ip], row[ip1], ap);
-        const float hi = lerpf(row[ia], row[ib], ap);
-        return lerpf(lo, hi, (KIND == K_VPHI) ? at : 0.5f * at);
+        ib = (ib == g.nP) ? 0 : ib;
+        // out-of-range rows (a backtrace past a pole) clamp to the field
+        const int rA = min(max(jt, 0), rows - 1);
+        const int rB = pole ? rA : min(max(jt + 1, 0), rows - 1);
+        const int cB0 = pole ? ia : ip, cB1 = pole ? ib : ip1;
+        w = pole ? ((KIND == K_VPHI) ? at : 0.5f * at) : at;
+        const int oA = __mul24(rA, g.nP), oB = __mul24(rB, g.nP);
+        a0 = ldf(f, oA + ip);
+        a1 = ldf(f, oA + ip1);
+        b0 = ldf(f, oB + cB0);
+        b1 = ldf(f, oB + cB1);
     }
-    const int r0 = min(max(jt, 0), rows - 1);
-    const int r1 = min(max(jt + 1, 0), rows - 1);
-    const float* row0 = f + (size_t)r0 * nPhi;
-    const float* row1 = f + (size_t)r1 * nPhi;
-    const float lo = lerpf(row0[ip], row0[ip1], ap);
-    const float hi = lerpf(row1[ip], row1[ip1], ap);
-    return lerpf(lo, hi, at);
+    return lerpf(lerpf(a0, a1, ap), lerpf(b0, b1, ap), w);
 }
 
 // Node offsets of each grid in cells (KaminoHeader.cuh:30-41) and the shift
@@ -101,23 +113,24 @@ __device__ constexpr float kShiftT[3] = {-0.5f, -1.f, -0.5f};
 // RK2 backtrace from node (i, j) of grid NODE, sampling field `src` of grid
 // SRC at the departure point (KaminoCore.cu:186-319).  cofP converts u_phi to
 // phi cells per step at this node's latitude, cofT u_theta to theta cells.
+// A field sampled at its own node is the node value (both offsets integral,
+// the ordinary branch with zero weights), so that read is direct.
 template <int NODE, int SRC>
-__device__ __forceinline__ float advect_node(const float* __restrict__ up,
-                                             const float* __restrict__ ut,
-                                             const float* __restrict__ src, int nTheta, int nPhi,
-                                             int i, int j, float cofP, float cofT) {
+__device__ __forceinline__ float advect_node(Rsrc up, Rsrc ut, Rsrc src, const Grid& g, int i, int j,
+                                             float cofP, float cofT) {
     constexpr float pP = kNodeP[NODE] + kShiftP[K_VPHI], pT = kNodeT[NODE] + kShiftT[K_VPHI];
     constexpr float tP = kNodeP[NODE] + kShiftP[K_VTHETA], tT = kNodeT[NODE] + kShiftT[K_VTHETA];
     constexpr float sP = kNodeP[NODE] + kShiftP[SRC], sT = kNodeT[NODE] + kShiftT[SRC];
-    const float u0p = sample<K_VPHI>(up, nTheta, nPhi, i, pP, j, pT);
-    const float u0t = sample<K_VTHETA>(ut, nTheta, nPhi, i, tP, j, tT);
+    const int o = __mul24(j, g.nP) + i;
+    const float u0p = (NODE == K_VPHI) ? ldf(up, o) : sample<K_VPHI>(up, g, i, pP, j, pT);
+    const float u0t = (NODE == K_VTHETA) ? ldf(ut, o) : sample<K_VTHETA>(ut, g, i, tP, j, tT);
     float dp = u0p * cofP;
     float dt = u0t * cofT;
-    const float ump = sample<K_VPHI>(up, nTheta, nPhi, i, pP - 0.5f * dp, j, pT - 0.5f * dt);
-    const float umt = sample<K_VTHETA>(ut, nTheta, nPhi, i, tP - 0.5f * dp, j, tT - 0.5f * dt);
+    const float ump = sample<K_VPHI>(up, g, i, pP - 0.5f * dp, j, pT - 0.5f * dt);
+    const float umt = sample<K_VTHETA>(ut, g, i, tP - 0.5f * dp, j, tT - 0.5f * dt);
     dp = 0.5f * (u0p + ump) * cofP;
     dt = 0.5f * (u0t + umt) * cofT;
-    return sample<SRC>(src, nTheta, nPhi, i, sP - dp, j, sT - dt);
+    return sample<SRC>(src, g, i, sP - dp, j, sT - dt);
 }
 
 // ---------------------------------------------------------------------------
@@ -168,13 +181,13 @@ __device__ __forceinline__ float geo_root(float G, float vb, float ub) {
     const float b = fmaf(G, vb, 1.f);
     if (b > 0.f) {
         const float a = G * G;
-        float x = ub / b;
+        float x = ub * __builtin_amdgcn_rcpf(b);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) {
+        for (int k = 0; k < 5; ++k) {
             const float x2 = x * x;
             const float f = fmaf(a * x2, x, fmaf(b, x, -ub));
             const float fp = fmaf(3.f * a, x2, b);
-            x -= f / fp;
+            x = fmaf(-f, __builtin_amdgcn_rcpf(fp), x);
         }
         return x;
     }
@@ -182,11 +195,17 @@ __device__ __forceinline__ float geo_root(float G, float vb, float ub) {
 }
 
 // ---------------------------------------------------------------------------
-// k_advect: one tile = rows [j0, j0 + R) of one simulation, full phi width.
-// With the geometric update on, the tile advects one extra u_phi row and two
-// extra u_theta rows into LDS (the update reads the rows either side), solves
-// the cubic per cell and restaggers (assignPhiKernel :517-534,
-// assignThetaKernel :537-549) before anything leaves the CU.
+// k_advect: one tile = rows [j0, j0 + R) x columns [c0, c0 + W) of one
+// simulation.  With the geometric update on, the tile advects into LDS the
+// u_phi values of rows j0..j0+R and columns c0-1..c0+W, and the u_theta
+// values of rows j0-1..j0+R and columns c0-1..c0+W-1 (the update reads one
+// row either side and one column to the west; the restaggered u_phi one more
+// column to the east), plus, in a polar tile, u_theta of the pole-adjacent
+// row at the antipodal columns.  It then solves the cubic per cell and
+// restaggers (assignPhiKernel :517-534, assignThetaKernel :537-549) before
+// anything leaves the CU.  Small tiles keep LDS near 15 KiB, so a CU holds
+// eight workgroups, which the latency of the gathers needs.
+// Flat item loops split an index into (row, column) with a magic multiply.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims,
                                                 const Tile* __restrict__ tiles, int ntiles,
@@ -196,85 +215,119 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const Tile tile = tiles[t];
         const SimDesc& s = sims[tile.sim];
-        const int nT = s.nTheta, nP = s.nPhi;
-        const float* __restrict__ up = s.state[parity][F_UPHI];
-        const float* __restrict__ ut = s.state[parity][F_UTHETA];
-        const float* __restrict__ rh = s.state[parity][F_RHO];
+        const Grid g{s.nTheta, s.nPhi, s.nPhi >> 1, s.invNP};
+        const int nT = g.nT, nP = g.nP;
+        const int N = nT * nP;
+        const Rsrc up = make_rsrc(s.state[parity][F_UPHI], N);
+        const Rsrc ut = make_rsrc(s.state[parity][F_UTHETA], N - nP);
+        const Rsrc rh = make_rsrc(s.state[parity][F_RHO], N);
         float* __restrict__ oup = s.state[parity ^ 1][F_UPHI];
         float* __restrict__ out = s.state[parity ^ 1][F_UTHETA];
         float* __restrict__ orh = s.state[parity ^ 1][F_RHO];
         const float* __restrict__ cofPC = s.rowf + RT_COFPHI_C * nT;
         const float* __restrict__ cofPT = s.rowf + RT_COFPHI_T * nT;
         const float cofT = s.cofTheta;
-        const int j0 = tile.row0, R = tile.rows;
+        const int j0 = tile.row0, R = tile.rows, c0 = tile.col0;
+        const int W = s.advCols, W1 = W + 1, W2 = W + 2;
+        const unsigned mW = s.magicW[0], mW1 = s.magicW[1], mW2 = s.magicW[2];
+        // tile column c (from -1) -> grid column, wrapping at the phi seam
+        auto col = [&](int c) {
+            int i = c0 + c;
+            i += (i < 0) ? nP : 0;
+            return i - ((i >= nP) ? nP : 0);
+        };
 
         if (!(s.flags & SIM_GEOMETRIC)) {
-            for (int idx = tid; idx < R * nP; idx += nthr) {
-                const int r = idx / nP, i = idx - r * nP, j = j0 + r;
-                const size_t o = (size_t)j * nP + i;
-                oup[o] = advect_node<K_VPHI, K_VPHI>(up, ut, up, nT, nP, i, j, cofPC[j], cofT);
-                if (j < nT - 1)
-                    out[o] = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, nT, nP, i, j, cofPT[j], cofT);
-                orh[o] = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, nT, nP, i, j, cofPC[j], cofT);
+            for (int idx = tid; idx < R * W; idx += nthr) {
+                const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
+                const int i = c0 + c, j = j0 + r, o = j * nP + i;
+                oup[o] = advect_node<K_VPHI, K_VPHI>(up, ut, up, g, i, j, cofPC[j], cofT);
+                if (j < nT - 1) out[o] = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, g, i, j, cofPT[j], cofT);
+                orh[o] = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, g, i, j, cofPC[j], cofT);
             }
             continue;
         }
 
         const int uLo = j0, uHi = min(j0 + R, nT - 1);
         const int vLo = max(j0 - 1, 0), vHi = min(j0 + R, nT - 2);
-        float* U = lds;                      // advected u_phi rows uLo..uHi
-        float* V = U + (size_t)(R + 1) * nP;  // advected u_theta rows vLo..vHi
-        float* X = V + (size_t)(R + 2) * nP;  // cubic roots x, rows j0..j0+R-1
-        const int nU = (uHi - uLo + 1) * nP, nV = (vHi - vLo + 1) * nP;
-        for (int idx = tid; idx < nU; idx += nthr) {
-            const int r = idx / nP, i = idx - r * nP, j = uLo + r;
-            U[idx] = advect_node<K_VPHI, K_VPHI>(up, ut, up, nT, nP, i, j, cofPC[j], cofT);
+        float* U = lds;                  // (R+1) x W2: u_phi*, columns c0-1 .. c0+W
+        float* V = U + (R + 1) * W2;     // (R+2) x W1: u_theta*, columns c0-1 .. c0+W-1
+        float* X = V + (R + 2) * W1;     // R x W1: cubic roots, columns c0-1 .. c0+W-1
+        float* A0 = X + R * W1;          // W1: u_theta* of row 0 at the antipodal columns
+        float* AL = A0 + W1;             // W1: u_theta* of row nT-2 at the antipodal columns
+        const bool north = (j0 == 0), south = (uHi == nT - 1);
+        const int half = g.half;
+        const int nU = (uHi - uLo + 1) * W2, nV = (vHi - vLo + 1) * W1, nR = R * W;
+        // two independent backtraces per iteration give the gathers more to overlap
+        for (int idx = tid; idx < nU; idx += 2 * nthr) {
+            const int idx2 = min(idx + nthr, nU - 1);
+            const int r = (int)__umulhi((unsigned)idx, mW2), i = col(idx - r * W2 - 1), j = uLo + r;
+            const int r2 = (int)__umulhi((unsigned)idx2, mW2), i2 = col(idx2 - r2 * W2 - 1), j2 = uLo + r2;
+            const float a = advect_node<K_VPHI, K_VPHI>(up, ut, up, g, i, j, cofPC[j], cofT);
+            const float b = advect_node<K_VPHI, K_VPHI>(up, ut, up, g, i2, j2, cofPC[j2], cofT);
+            U[idx] = a;
+            U[idx2] = b;
         }
-        for (int idx = tid; idx < nV; idx += nthr) {
-            const int r = idx / nP, i = idx - r * nP, j = vLo + r;
-            V[idx] = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, nT, nP, i, j, cofPT[j], cofT);
+        for (int idx = tid; idx < nV; idx += 2 * nthr) {
+            const int idx2 = min(idx + nthr, nV - 1);
+            const int r = (int)__umulhi((unsigned)idx, mW1), i = col(idx - r * W1 - 1), j = vLo + r;
+            const int r2 = (int)__umulhi((unsigned)idx2, mW1), i2 = col(idx2 - r2 * W1 - 1), j2 = vLo + r2;
+            const float a = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, g, i, j, cofPT[j], cofT);
+            const float b = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, g, i2, j2, cofPT[j2], cofT);
+            V[idx] = a;
+            V[idx2] = b;
         }
-        for (int idx = tid; idx < R * nP; idx += nthr) {
-            const int r = idx / nP, i = idx - r * nP, j = j0 + r;
-            orh[(size_t)j * nP + i] =
-                advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, nT, nP, i, j, cofPC[j], cofT);
+        if (north || south) {
+            for (int idx = tid; idx < 2 * W1; idx += nthr) {
+                const bool s0 = idx < W1;
+                if (s0 ? !north : !south) continue;
+                const int c = s0 ? idx : idx - W1;
+                int io = col(c - 1) + half;
+                io -= (io >= nP) ? nP : 0;
+                const int j = s0 ? 0 : nT - 2;
+                const float a = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, g, io, j, cofPT[j], cofT);
+                (s0 ? A0 : AL)[c] = a;
+            }
+        }
+        for (int idx = tid; idx < nR; idx += 2 * nthr) {
+            const int idx2 = min(idx + nthr, nR - 1);
+            const int r = (int)__umulhi((unsigned)idx, mW), i = c0 + idx - r * W, j = j0 + r;
+            const int r2 = (int)__umulhi((unsigned)idx2, mW), i2 = c0 + idx2 - r2 * W, j2 = j0 + r2;
+            const float a = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, g, i, j, cofPC[j], cofT);
+            const float b = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, g, i2, j2, cofPC[j2], cofT);
+            orh[j * nP + i] = a;
+            orh[j2 * nP + i2] = b;
         }
         __syncthreads();
 
-        // geometricFillKernel per cell, walking down each owned column so that
-        // v' of the row above stays in a register for the u_theta restagger
+        // geometricFillKernel per cell, walking down each column so that v'
+        // of the row above stays in a register for the u_theta restagger
         const float* __restrict__ Gt = s.rowf + RT_G * nT;
-        const int half = nP >> 1;
-        for (int i = tid; i < nP; i += nthr) {
-            const int ir = (i + 1 == nP) ? 0 : i + 1;
-            const int io = (i + half) % nP;
+        for (int c = tid; c < W1; c += nthr) {
+            const int i = col(c - 1);
             float vprev = 0.f;
             for (int j = j0; j <= uHi; ++j) {
-                const float* Ur = U + (size_t)(j - uLo) * nP;
-                const float ub = 0.5f * (Ur[i] + Ur[ir]);
+                const float* Ur = U + (j - uLo) * W2;
+                const float ub = 0.5f * (Ur[c] + Ur[c + 1]);
                 float vb;
-                if (j == 0) {
-                    const float* V0 = V + (size_t)(0 - vLo) * nP;
-                    vb = 0.75f * V0[i] + 0.25f * V0[io];
-                } else if (j == nT - 1) {
-                    const float* Vl = V + (size_t)(nT - 2 - vLo) * nP;
-                    vb = 0.75f * Vl[i] + 0.25f * Vl[io];
-                } else {
-                    vb = 0.5f * (V[(size_t)(j - 1 - vLo) * nP + i] + V[(size_t)(j - vLo) * nP + i]);
-                }
+                if (j == 0)
+                    vb = 0.75f * V[(0 - vLo) * W1 + c] + 0.25f * A0[c];
+                else if (j == nT - 1)
+                    vb = 0.75f * V[(nT - 2 - vLo) * W1 + c] + 0.25f * AL[c];
+                else
+                    vb = 0.5f * (V[(j - 1 - vLo) * W1 + c] + V[(j - vLo) * W1 + c]);
                 const float G = Gt[j];
                 const float x = geo_root(G, vb, ub);
                 const float vp = fmaf(G * x, x, vb);
-                if (j < j0 + R) X[(size_t)(j - j0) * nP + i] = x;
-                if (j > j0) out[(size_t)(j - 1) * nP + i] = 0.5f * (vprev + vp);
+                if (j < j0 + R) X[(j - j0) * W1 + c] = x;
+                if (j > j0 && c > 0) out[(j - 1) * nP + i] = 0.5f * (vprev + vp);
                 vprev = vp;
             }
         }
         __syncthreads();
-        for (int idx = tid; idx < R * nP; idx += nthr) {
-            const int r = idx / nP, i = idx - r * nP;
-            const int il = (i == 0) ? nP - 1 : i - 1;
-            oup[(size_t)(j0 + r) * nP + i] = 0.5f * (X[(size_t)r * nP + il] + X[idx]);
+        for (int idx = tid; idx < nR; idx += nthr) {
+            const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
+            oup[(j0 + r) * nP + c0 + c] = 0.5f * (X[r * W1 + c] + X[r * W1 + c + 1]);
         }
         __syncthreads();
     }

@@ -29,6 +29,7 @@ struct Plan {
     int nfac = 0;
     int fac[12] = {0};
     int advRows = 0;   // rows per advection tile
+    int advCols = 0;   // columns per advection tile
     int fftRows = 0;   // rows per FFT tile
     int triL = 0;      // lanes per mode in the tridiagonal kernel
     int triK = 0;      // template row capacity per lane
@@ -90,5 +91,7 @@ private:
 enum KernelCat { KC_ADVECT = 0, KC_DIVFFT = 1, KC_TRI = 2, KC_IFFT = 3, KC_COUNT = 4 };
 
 int ensure_device(int device);
+// Compute units of the last device made ready (for tile-count heuristics).
+int device_cus();
 
 }  // namespace kam

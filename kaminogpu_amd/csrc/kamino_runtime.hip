@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -49,7 +50,10 @@ static int factor_m(int M, int* fac) {
     return M == 1 ? n : -1;
 }
 
-static size_t adv_lds(int R, int nP) { return (size_t)(3 * R + 3) * nP * sizeof(float); }
+// LDS of a geometric advection tile of R rows and W columns (k_advect)
+static size_t adv_lds(int R, int W) {
+    return (size_t)((R + 1) * (W + 2) + (R + 2) * (W + 1) + R * (W + 1) + 2 * (W + 1)) * sizeof(float);
+}
 static size_t fft_lds(int R, int M) { return (size_t)(R + 1) * M * 2 * sizeof(float); }
 
 int make_plan(int nT, int nP, Plan& p) {
@@ -63,11 +67,10 @@ int make_plan(int nT, int nP, Plan& p) {
     p.nfac = factor_m(p.M, p.fac);
     if (p.nfac < 0)
         return set_err(KAM_EINVAL, "n_theta must factor as 2^a 3^b 5^c (got %d)", nT);
-    // advection tile: LDS (3R+3) rows of nPhi floats, kept to 64 KiB where possible
-    const size_t cap = nP <= 1024 ? 64 * 1024 : 150 * 1024;
-    int R = (int)(cap / (12 * (size_t)nP)) - 1;
-    R = R < 1 ? 1 : (R > 8 ? 8 : R);
-    p.advRows = R > nT ? nT : R;
+    // advection tile: 8 rows x 128 columns (about 15 KiB of LDS, so eight
+    // workgroups fit a CU); narrower spheres use their full width
+    p.advCols = nP < 128 ? nP : 128;
+    p.advRows = nT < 8 ? nT : 8;
     // FFT tile: R * M <= 4096 complex, plus one halo row for the inverse
     int Rf = 4096 / p.M;
     Rf = Rf < 1 ? 1 : Rf;
@@ -80,16 +83,23 @@ int make_plan(int nT, int nP, Plan& p) {
     else if (nT <= 256) { p.triL = 32; p.triK = 8; }
     else if (nT <= 512) { p.triL = 64; p.triK = 8; }
     else { p.triL = 64; p.triK = 16; }
-    p.advLds = adv_lds(p.advRows, nP);
+    // manual overrides for tile sweeps
+    if (const char* v = std::getenv("KAM_ADV_COLS")) {
+        const int w = std::atoi(v);
+        if (w >= 8 && w <= 254) p.advCols = w < nP ? w : nP;
+    }
+    if (const char* v = std::getenv("KAM_ADV_ROWS")) adjust_plan(p, std::atoi(v), 0);
+    if (const char* v = std::getenv("KAM_FFT_ROWS")) adjust_plan(p, 0, std::atoi(v));
+    p.advLds = adv_lds(p.advRows, p.advCols);
     p.fftLds = fft_lds(p.fftRows, p.M);
     return KAM_OK;
 }
 
 int adjust_plan(Plan& p, int advRows, int fftRows) {
     if (advRows > 0) {
-        if (adv_lds(advRows, p.nP) > 160 * 1024) return set_err(KAM_EINVAL, "advection tile too tall");
+        if (advRows > 64) return set_err(KAM_EINVAL, "advection tile too tall");
         p.advRows = advRows > p.nT ? p.nT : advRows;
-        p.advLds = adv_lds(p.advRows, p.nP);
+        p.advLds = adv_lds(p.advRows, p.advCols);
     }
     if (fftRows > 0) {
         if ((size_t)(fftRows + 1) * p.M > 7680) return set_err(KAM_EINVAL, "FFT tile too tall");
@@ -189,6 +199,13 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
     for (int k = 0; k < p.nfac; ++k) d.fac[k] = p.fac[k];
     d.advRows = p.advRows;
     d.fftRows = p.fftRows;
+    d.magicP = (unsigned)((((uint64_t)1 << 32) + (uint64_t)p.nP - 1) / (uint64_t)p.nP);
+    d.invNP = 1.0f / (float)p.nP;
+    d.advCols = p.advCols;
+    for (int k = 0; k < 3; ++k) {
+        const uint64_t w = (uint64_t)p.advCols + k;
+        d.magicW[k] = (unsigned)((((uint64_t)1 << 32) + w - 1) / w);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -217,26 +234,56 @@ int DeviceBatch::reserve(size_t ns, size_t nt) {
     return KAM_OK;
 }
 
+// Tile heights: each simulation's planned height, halved for every
+// simulation while a kernel would get fewer than two workgroups per CU.  The
+// height changes only how rows are grouped, never a value.
+static int shrink_until_busy(const std::vector<Plan>& plans, std::vector<int>& rows, int minTiles,
+                             bool advection) {
+    for (;;) {
+        long tiles = 0;
+        bool canShrink = false;
+        for (size_t s = 0; s < plans.size(); ++s) {
+            const long cols = advection ? (plans[s].nP + plans[s].advCols - 1) / plans[s].advCols : 1;
+            tiles += cols * ((plans[s].nT + rows[s] - 1) / rows[s]);
+            canShrink |= rows[s] > 1;
+        }
+        if (tiles >= minTiles || !canShrink) return (int)tiles;
+        for (int& r : rows) r = r > 1 ? (r + 1) / 2 : 1;
+    }
+}
+
 int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>& plans,
                        hipStream_t st) {
     hTiles_.clear();
     tri_.clear();
     advLds_ = fftLds_ = 0;
-    // advection tiles, tallest simulations first for load balance
+    const int minTiles = 2 * device_cus();
+    std::vector<int> advRows(plans.size()), fftRows(plans.size());
+    for (size_t s = 0; s < plans.size(); ++s) {
+        advRows[s] = plans[s].advRows;
+        fftRows[s] = plans[s].fftRows;
+    }
+    shrink_until_busy(plans, advRows, minTiles, true);
+    shrink_until_busy(plans, fftRows, minTiles, false);
+    // advection tiles, in batch order (tallest simulations first); the last
+    // column tile of a row band overlaps its neighbour instead of running short
     advOff_ = 0;
     for (size_t s = 0; s < sims.size(); ++s) {
         const Plan& p = plans[s];
-        for (int j = 0; j < p.nT; j += p.advRows)
-            hTiles_.push_back(Tile{(int)s, j, std::min(p.advRows, p.nT - j), 0});
-        if (sims[s].flags & SIM_GEOMETRIC) advLds_ = std::max(advLds_, p.advLds);
+        for (int j = 0; j < p.nT; j += advRows[s])
+            for (int c = 0; c < p.nP; c += p.advCols)
+                hTiles_.push_back(Tile{(int)s, j, std::min(advRows[s], p.nT - j),
+                                       std::min(c, p.nP - p.advCols)});
+        if (sims[s].flags & SIM_GEOMETRIC)
+            advLds_ = std::max(advLds_, adv_lds(advRows[s], p.advCols));
     }
     nAdv_ = (int)hTiles_.size();
     fftOff_ = (int)hTiles_.size();
     for (size_t s = 0; s < sims.size(); ++s) {
         const Plan& p = plans[s];
-        for (int j = 0; j < p.nT; j += p.fftRows)
-            hTiles_.push_back(Tile{(int)s, j, std::min(p.fftRows, p.nT - j), 0});
-        fftLds_ = std::max(fftLds_, p.fftLds);
+        for (int j = 0; j < p.nT; j += fftRows[s])
+            hTiles_.push_back(Tile{(int)s, j, std::min(fftRows[s], p.nT - j), 0});
+        fftLds_ = std::max(fftLds_, (size_t)(fftRows[s] + 1) * p.M * 2 * sizeof(float));
     }
     nFft_ = (int)hTiles_.size() - fftOff_;
     // tridiagonal tiles, grouped by kernel class
@@ -306,6 +353,9 @@ int DeviceBatch::launch(int phases, int parity, hipStream_t st, int* newParity,
 // ---------------------------------------------------------------------------
 static std::mutex g_dev_mu;
 static std::map<int, bool> g_dev_ready;
+static int g_cus = 256;
+
+int device_cus() { return g_cus; }
 
 int ensure_device(int device) {
     std::lock_guard<std::mutex> lk(g_dev_mu);
@@ -325,6 +375,7 @@ int ensure_device(int device) {
                        prop.gcnArchName);
     KAM_HIP(hipSetDevice(device));
     KAM_HIP(configure_kernels(160 * 1024));
+    g_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     g_dev_ready[device] = true;
     return KAM_OK;
 }
