@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this size (for HBM counter scaling)")
     return ap.parse_args()
 
 
@@ -74,6 +76,23 @@ def cpu_baseline(nt, dt, seconds):
     }
 
 
+def copy_calibration(nbytes):
+    """One device fill and one device-to-device copy of nbytes through the HIP
+    runtime: kernels of known traffic that scale the HBM byte counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {nbytes} bytes filled, then copied device to device")
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -86,6 +105,8 @@ def main():
         raise SystemExit("no gfx950 device visible: " + K.lib().kam_last_error().decode())
     nt, dt, B = a.n_theta, 0.002, a.batch
     cells = nt * 2 * nt
+    if a.copy_calibration_mib > 0:
+        copy_calibration(a.copy_calibration_mib << 20)
 
     # ---- plain form: one simulation, per-step kernel sequence, no graph
     plain, p_steps = None, np.zeros(1)
