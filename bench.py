@@ -76,6 +76,21 @@ def cpu_baseline(nt, dt, seconds):
     }
 
 
+def measured_traffic(kernel, nt, batch):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary, scaled by the copy-kernel calibration; None when the
+    summary is for another workload."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{batch}x{nt}x{2 * nt}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    name = {"advect": "k_advect", "div_fft": "k_div_fft", "tridiag": "k_tridiag",
+            "ifft_grad": "k_ifft_grad"}[kernel]
+    k = t["kernels"].get(name)
+    return None if k is None else k["hbm_bytes"]
+
+
 def copy_calibration(nbytes):
     """One device fill and one device-to-device copy of nbytes through the HIP
     runtime: kernels of known traffic that scale the HBM byte counters."""
@@ -140,6 +155,7 @@ def main():
     dom = max(kern, key=kern.get)
     gbs = {k: BYTES_PER_CELL[k] * cells * B / (kern[k] * 1e-3) / 1e9 for k in KERNELS}
     achieved = gbs[dom]
+    traffic = measured_traffic(dom, nt, B)
     out = {
         "metric": METRIC,
         "value": batched,
@@ -163,7 +179,8 @@ def main():
         "kernel_ms": kern,
         "kernel_gbs": gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": BYTES_PER_CELL[dom] * cells * B},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nt, dt, a.cpu_seconds)

@@ -37,6 +37,16 @@ __device__ __forceinline__ float ldf(Rsrc r, int elem) {
 }
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Block b takes tile (b % 8) * per + b / 8, so every XCD walks a contiguous
+// run of the tile list and neighbouring tiles, which share halo rows and
+// spectrum lines, meet in the same L2.  Launches use 8 * per blocks.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+    const int per = (ntiles + kXcds - 1) / kXcds;
+    return (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+}
+
 // Per-shape constants a sampler needs.
 struct Grid {
     int nT, nP, half;
@@ -212,7 +222,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
                                                 int parity) {
     extern __shared__ float lds[];
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
         const SimDesc& s = sims[tile.sim];
         const Grid g{s.nTheta, s.nPhi, s.nPhi >> 1, s.invNP};
@@ -467,7 +477,7 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
                                                  int parity) {
     extern __shared__ float2 zbuf[];
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
         const SimDesc& s = sims[tile.sim];
         const int nT = s.nTheta, nP = s.nPhi, M = s.M;
@@ -550,7 +560,7 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
     const int g = tid / L;               // mode within the block
     const int wl = threadIdx.x & 63;     // lane within the wavefront
     const int base = wl - l;             // first wavefront lane of this group
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
         const SimDesc& s = sims[tile.sim];
         const int nT = s.nTheta;
@@ -713,7 +723,7 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
                                                    int parity) {
     extern __shared__ float2 zbuf[];
     const int tid = threadIdx.x, nthr = blockDim.x;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
         const SimDesc& s = sims[tile.sim];
         const int nT = s.nTheta, nP = s.nPhi, M = s.M;
@@ -760,7 +770,11 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
 // ---------------------------------------------------------------------------
 // host-side launchers
 // ---------------------------------------------------------------------------
-static inline int grid_for(int ntiles, int cap) { return ntiles < cap ? ntiles : cap; }
+// one block per tile, rounded up to a multiple of the XCD count (see xcd_tile)
+static inline int grid_for(int ntiles, int cap) {
+    (void)cap;
+    return ((ntiles + kXcds - 1) / kXcds) * kXcds;
+}
 
 hipError_t launch_advect(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
                          size_t ldsBytes, int gridCap, hipStream_t st) {
