@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--uneven-steps", type=int, default=200, help="config-3 batch steps (0 skips)")
+    ap.add_argument("--serve-requests", type=int, default=2000, help="config-4 requests (0 skips)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (for HBM counter scaling)")
     return ap.parse_args()
@@ -49,10 +51,9 @@ def log(msg):
 
 
 def init_state(nt, L, seed, dens):
-    from oracle import oracle as O
+    import kaminogpu_amd as K
 
-    up, ut, rho = O.init_fields(nt, 2 * nt, L, seed, dens)
-    return [np.ascontiguousarray(a, np.float32) for a in (up, ut, rho)]
+    return list(K.init_fields(nt, L, seed, dens))
 
 
 def cpu_baseline(nt, dt, seconds):
@@ -74,6 +75,42 @@ def cpu_baseline(nt, dt, seconds):
         "sample": f"{steps} FP64 steps of one {nt}x{2 * nt} sphere (dt {dt}) on {threads} OpenMP "
                   f"threads, {model}".strip(),
     }
+
+
+def run_uneven(K, steps, sims=64, seed=3):
+    """Config 3: 64 spheres with n_theta drawn from {32,...,256}, dt = 0.5/n_theta,
+    L = min(16, n_theta/4), density band, stepped together as graph replays."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([32, 48, 64, 96, 128, 192, 256], sims)
+    with K.Engine(use_graph=True) as eng:
+        for k, nt in enumerate(sizes):
+            st = K.init_fields(int(nt), min(16, int(nt) // 4), 100 + k, K.DENSITY_BAND)
+            eng.admit(int(nt), 0.5 / nt, *st, steps=steps + 20)
+        eng.step(10)
+        eng.reset_stats()
+        eng.step(steps)
+        s = eng.stats()
+    v = s.cell_steps / (s.gpu_ms * 1e-3)
+    log(f"uneven x{sims}: {v:.3e} cell-steps/s, p50 {s.step_p50_ms:.4f} p99 {s.step_p99_ms:.4f} ms")
+    return {"sims": sims, "cells": int(sum(2 * int(n) ** 2 for n in sizes)), "steps": steps,
+            "cell_steps_per_s": v, "p50_step_ms": s.step_p50_ms, "p99_step_ms": s.step_p99_ms}
+
+
+def run_served(K, n, batched_rate):
+    """Config 4: Poisson arrivals at ~80% of the batched capacity; n_theta
+    log-uniform in [32, 512] (rounded to supported sizes), steps U{50..2000}."""
+    from kaminogpu_amd import serve
+
+    reqs = serve.make_requests(n, seed=4)
+    work = float(np.mean([2 * r.n_theta ** 2 * r.steps for r in reqs]))
+    rate = 0.8 * batched_rate / work
+    with K.Engine(use_graph=True) as eng:
+        out = serve.serve(reqs, rate, eng, log=log)
+    log(f"served {n}: {out['cell_steps_per_s']:.3e} cell-steps/s, step p50 {out['step_p50_ms']:.3f} "
+        f"p99 {out['step_p99_ms']:.3f} ms, request p50 {out['request_p50_s']:.3f} p99 "
+        f"{out['request_p99_s']:.3f} s")
+    out["mean_cell_steps_per_request"] = work
+    return out
 
 
 def measured_traffic(kernel, nt, batch):
@@ -151,6 +188,9 @@ def main():
     log(f"batched x{B}: {batched:.3e} cell-steps/s, {ms_per_step:.4f} ms/step, "
         f"p50 {st.step_p50_ms:.4f} p99 {st.step_p99_ms:.4f}")
 
+    uneven = run_uneven(K, a.uneven_steps) if a.uneven_steps > 0 else None
+    served = run_served(K, a.serve_requests, batched) if a.serve_requests > 0 else None
+
     kern = {k: float(v) for k, v in zip(KERNELS, kern_ms)}
     dom = max(kern, key=kern.get)
     gbs = {k: BYTES_PER_CELL[k] * cells * B / (kern[k] * 1e-3) / 1e9 for k in KERNELS}
@@ -176,6 +216,8 @@ def main():
         "plain": {"cell_steps_per_s": plain, "steps": a.plain_steps,
                   "p50_step_ms": float(np.percentile(p_steps, 50)),
                   "p99_step_ms": float(np.percentile(p_steps, 99))},
+        "uneven_batch": uneven,
+        "served": served,
         "kernel_ms": kern,
         "kernel_gbs": gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -120,6 +120,16 @@ int kam_sim_sync(kam_sim *sim);
 /* Bytes of device memory one simulation of this shape needs. */
 size_t kam_sim_bytes(int n_theta, int n_phi);
 
+/* Seeded synthetic initial state (host, FP32): the discrete curl of a real
+ * spherical-harmonic streamfunction with coefficients c_lm ~ N(0, 0.1/l),
+ * l = 1..L, drawn by SplitMix64 + Box-Muller, and a density of
+ * 0.5 + 0.1 U[0,1) (density_kind 0) or a band around the equator (1).  No
+ * counterpart in the reference, whose initialisers are FBM noise and images
+ * (KaminoInitializer.cu:3-134, KaminoSolver.cu:243-277).  Any output may be
+ * NULL. */
+int kam_init_fields(int n_theta, int n_phi, double radius, int L, uint64_t seed,
+                    int density_kind, float *u_phi, float *u_theta, float *rho);
+
 /* ==== Batching and serving engine ========================================
  * No counterpart in the reference, which runs one simulation per process
  * (KaminoSolver.cu:20 hard-codes device 0 and the dtor resets the device).
@@ -138,7 +148,7 @@ typedef struct kam_engine_desc {
     int max_slots;            /* concurrently active simulations (0 = 4096) */
     size_t hbm_budget_bytes;  /* arena for simulation state (0 = half of free HBM) */
     int use_graph;            /* 1: replay each step as a hipGraph */
-    int reserved;
+    int max_inflight;         /* steps queued ahead of the GPU at most (0 = 2) */
 } kam_engine_desc;
 
 enum { KAM_REQ_ACTIVE = 1, KAM_REQ_DONE = 2, KAM_REQ_CANCELLED = 3, KAM_REQ_FAILED = 4 };
@@ -179,6 +189,10 @@ int kam_engine_admit(kam_engine *eng, const kam_sim_desc *desc, const float *u_p
 int kam_engine_step(kam_engine *eng, int nsteps);
 
 int kam_engine_poll(kam_engine *eng, uint64_t id, kam_req_status *status);
+
+/* Ids of requests whose final fields reached the host since the last call
+ * (up to max; *count receives how many were written). */
+int kam_engine_completed(kam_engine *eng, uint64_t *ids, int max, int *count);
 
 /* Copies a finished request's fields (waits for its copy) and forgets it. */
 int kam_engine_fetch(kam_engine *eng, uint64_t id, float *u_phi, float *u_theta, float *rho);

@@ -29,7 +29,7 @@ ABI_SYMBOLS = [
     "kam_last_error", "kam_version", "kam_device_count",
     "kam_sim_create", "kam_sim_destroy", "kam_sim_set_fields", "kam_sim_get_fields",
     "kam_sim_step", "kam_sim_step_phases", "kam_sim_get_spectrum", "kam_sim_sync",
-    "kam_sim_bytes", "kam_sim_benchmark",
+    "kam_sim_bytes", "kam_sim_benchmark", "kam_init_fields", "kam_engine_completed",
     "kam_engine_create", "kam_engine_destroy", "kam_engine_admit", "kam_engine_step",
     "kam_engine_poll", "kam_engine_fetch", "kam_engine_cancel", "kam_engine_sync",
     "kam_engine_stats", "kam_engine_reset_stats", "kam_engine_profile", "kam_engine_now",
@@ -76,6 +76,7 @@ def lib():
     L.kam_sim_benchmark.argtypes = [C.c_void_p, C.c_int, C.c_int, _dpp, _dpp, _dpp]
     L.kam_sim_bytes.restype = C.c_size_t
     L.kam_sim_bytes.argtypes = [C.c_int, C.c_int]
+    L.kam_init_fields.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, C.c_int, _fp, _fp, _fp]
     _bind_engine(L)
     _lib = L
     return L
@@ -106,6 +107,20 @@ def _ptr(a):
 
 def device_count() -> int:
     return lib().kam_device_count()
+
+
+DENSITY_NOISE, DENSITY_BAND = 0, 1
+
+
+def init_fields(n_theta, L, seed, density=DENSITY_BAND, radius=1.0):
+    """Seeded synthetic state (kam_init_fields): FP32 (u_phi, u_theta, rho)."""
+    nt, nph = int(n_theta), 2 * int(n_theta)
+    up = np.empty((nt, nph), np.float32)
+    ut = np.empty((nt - 1, nph), np.float32)
+    rho = np.empty((nt, nph), np.float32)
+    check(lib().kam_init_fields(nt, nph, float(radius), int(L), int(seed), int(density), _ptr(up),
+                                _ptr(ut), _ptr(rho)))
+    return up, ut, rho
 
 
 class Sim:

@@ -108,6 +108,7 @@ struct Request {
     size_t off = 0, bytes = 0;
     int64_t total = 0, done = 0;
     int state = KAM_REQ_ACTIVE;
+    bool copying = false;  // retired; final copy to the host still in flight
     double admit_s = 0, done_s = 0;
     void* host = nullptr;  // pinned copy of the final fields
     size_t hostBytes = 0;
@@ -125,11 +126,15 @@ struct kam_engine {
     PinnedPool pinned;
     DeviceBatch batch;
     std::unordered_map<uint64_t, Request> reqs;
-    std::vector<uint64_t> active;  // batch order
+    std::vector<uint64_t> active;     // batch order
+    std::deque<uint64_t> draining;    // retired, final copy in flight (retirement order)
+    std::vector<uint64_t> completed;  // final copy landed, not yet reported
     uint64_t nextId = 1;
     int parity = 0;
     bool dirty = true;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // replaced graphs wait here until their last launch has finished
+    std::deque<std::pair<hipGraphExec_t, hipEvent_t>> oldGraphs;
     // statistics
     std::deque<hipEvent_t> stepEvents;  // recorded after each step, not yet collected
     std::vector<hipEvent_t> eventPool;
@@ -137,7 +142,7 @@ struct kam_engine {
     bool windowOpen = false;
     std::vector<float> stepMs;
     int64_t steps = 0, cellSteps = 0, curCells = 0;
-    int64_t admitted = 0, completed = 0, rejected = 0, cancelled = 0;
+    int64_t admitted = 0, ncompleted = 0, rejected = 0, cancelled = 0;
     std::chrono::steady_clock::time_point t0;
 
     double now() const {
@@ -147,13 +152,29 @@ struct kam_engine {
 
 namespace {
 
+// Retires the current graphs.  A graph exec owns its kernel arguments, so it
+// is destroyed only once an event queued after its last launch has completed.
 int drop_graphs(kam_engine* e) {
     for (auto& g : e->graph)
         if (g) {
-            KAM_HIP(hipGraphExecDestroy(g));
+            hipEvent_t ev = nullptr;
+            KAM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            KAM_HIP(hipEventRecord(ev, e->stream));
+            e->oldGraphs.emplace_back(g, ev);
             g = nullptr;
         }
     return KAM_OK;
+}
+
+void reap_graphs(kam_engine* e, bool wait) {
+    while (!e->oldGraphs.empty()) {
+        auto& og = e->oldGraphs.front();
+        if (wait) (void)hipEventSynchronize(og.second);
+        else if (hipEventQuery(og.second) != hipSuccess) break;
+        (void)hipGraphExecDestroy(og.first);
+        (void)hipEventDestroy(og.second);
+        e->oldGraphs.pop_front();
+    }
 }
 
 int rebuild(kam_engine* e) {
@@ -224,7 +245,8 @@ int retire(kam_engine* e, Request& r, int state) {
                                    hipMemcpyDeviceToHost, e->stream));
         KAM_HIP(hipEventCreateWithFlags(&r.copied, hipEventDisableTiming));
         KAM_HIP(hipEventRecord(r.copied, e->stream));
-        r.done_s = e->now();
+        r.copying = true;
+        e->draining.push_back(r.id);
     }
     r.state = state;
     e->arena.release(r.off, r.bytes);
@@ -234,7 +256,25 @@ int retire(kam_engine* e, Request& r, int state) {
     return KAM_OK;
 }
 
+// Marks retired requests whose final copy has landed; stamps their time.
+void drain(kam_engine* e) {
+    while (!e->draining.empty()) {
+        auto it = e->reqs.find(e->draining.front());
+        if (it != e->reqs.end()) {
+            Request& r = it->second;
+            if (hipEventQuery(r.copied) != hipSuccess) break;
+            r.copying = false;
+            r.done_s = e->now();
+            e->completed.push_back(r.id);
+        }
+        e->draining.pop_front();
+    }
+}
+
 int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
+    // keep at most max_inflight steps queued ahead of the GPU
+    const size_t q = e->stepEvents.size(), lim = (size_t)e->ed.max_inflight;
+    if (q >= lim) KAM_HIP(hipEventSynchronize(e->stepEvents[q - lim]));
     if (e->dirty) {
         int rc = rebuild(e);
         if (rc != KAM_OK) return rc;
@@ -276,8 +316,10 @@ int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
     for (uint64_t id : finished) {
         int rc = retire(e, e->reqs[id], KAM_REQ_DONE);
         if (rc != KAM_OK) return rc;
-        e->completed++;
+        e->ncompleted++;
     }
+    drain(e);
+    reap_graphs(e, false);
     if (e->stepEvents.size() > 1024) return collect(e, false);
     return KAM_OK;
 }
@@ -294,6 +336,7 @@ int kam_engine_create(const kam_engine_desc* d, kam_engine** out) {
     kam_engine* e = new kam_engine();
     e->ed = *d;
     if (e->ed.max_slots <= 0) e->ed.max_slots = 4096;
+    if (e->ed.max_inflight <= 0) e->ed.max_inflight = 2;
     e->t0 = std::chrono::steady_clock::now();
     size_t budget = d->hbm_budget_bytes;
     if (budget == 0) {
@@ -318,6 +361,7 @@ int kam_engine_destroy(kam_engine* e) {
     (void)hipSetDevice(e->ed.device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
+    reap_graphs(e, true);
     for (auto& kv : e->reqs) {
         Request& r = kv.second;
         if (r.state == KAM_REQ_ACTIVE) release_tables(e->ed.device, r.sd.n_theta, r.sd.dt, r.sd.radius);
@@ -418,10 +462,9 @@ int kam_engine_poll(kam_engine* e, uint64_t id, kam_req_status* st) {
     if (!e || !st) return set_err(KAM_EINVAL, "null argument");
     auto it = e->reqs.find(id);
     if (it == e->reqs.end()) return set_err(KAM_ENOTFOUND, "unknown request %llu", (unsigned long long)id);
+    drain(e);
     const Request& r = it->second;
-    st->state = r.state;
-    if (r.state == KAM_REQ_DONE && r.copied && hipEventQuery(r.copied) != hipSuccess)
-        st->state = KAM_REQ_ACTIVE;  // the final copy is still in flight
+    st->state = (r.state == KAM_REQ_DONE && r.copying) ? KAM_REQ_ACTIVE : r.state;
     st->reserved = 0;
     st->steps_done = r.done;
     st->steps_total = r.total;
@@ -466,7 +509,19 @@ int kam_engine_cancel(kam_engine* e, uint64_t id) {
 int kam_engine_sync(kam_engine* e) {
     if (!e) return set_err(KAM_EINVAL, "null engine");
     KAM_HIP(hipSetDevice(e->ed.device));
-    return collect(e, true);
+    int rc = collect(e, true);
+    drain(e);
+    return rc;
+}
+
+int kam_engine_completed(kam_engine* e, uint64_t* ids, int max, int* count) {
+    if (!e || !count || (max > 0 && !ids)) return set_err(KAM_EINVAL, "null argument");
+    drain(e);
+    const int n = std::min<int>(max, (int)e->completed.size());
+    for (int k = 0; k < n; ++k) ids[k] = e->completed[k];
+    e->completed.erase(e->completed.begin(), e->completed.begin() + n);
+    *count = n;
+    return KAM_OK;
 }
 
 int kam_engine_stats(kam_engine* e, kam_stats* s) {
@@ -493,7 +548,7 @@ int kam_engine_stats(kam_engine* e, kam_stats* s) {
         s->step_max_ms = v.back();
     }
     s->admitted = e->admitted;
-    s->completed = e->completed;
+    s->completed = e->ncompleted;
     s->rejected = e->rejected;
     s->cancelled = e->cancelled;
     s->bytes_in_use = e->arena.used();
@@ -516,7 +571,7 @@ int kam_engine_reset_stats(kam_engine* e) {
     e->stepMs.clear();
     e->steps = 0;
     e->cellSteps = 0;
-    e->admitted = e->completed = e->rejected = e->cancelled = 0;
+    e->admitted = e->ncompleted = e->rejected = e->cancelled = 0;
     return KAM_OK;
 }
 

@@ -76,10 +76,14 @@ public:
     size_t capacityBytes() const { return capSims_ * sizeof(SimDesc) + capTiles_ * sizeof(Tile); }
 
 private:
-    int reserve(size_t nsims, size_t ntiles);
+    int reserve(size_t nsims, size_t ntiles, hipStream_t st);
     SimDesc* dSims_ = nullptr;
     Tile* dTiles_ = nullptr;
     size_t capSims_ = 0, capTiles_ = 0;
+    // pinned staging of the last upload; `uploaded_` marks when the copy is done
+    void* hStage_ = nullptr;
+    size_t stageBytes_ = 0;
+    hipEvent_t uploaded_ = nullptr;
     std::vector<Tile> hTiles_;
     int nsims_ = 0;
     int advOff_ = 0, nAdv_ = 0, fftOff_ = 0, nFft_ = 0;

@@ -212,25 +212,40 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
 // Device batches
 // ---------------------------------------------------------------------------
 DeviceBatch::~DeviceBatch() {
+    if (uploaded_) (void)hipEventSynchronize(uploaded_);
     if (dSims_) (void)hipFree(dSims_);
     if (dTiles_) (void)hipFree(dTiles_);
+    if (hStage_) (void)hipHostFree(hStage_);
+    if (uploaded_) (void)hipEventDestroy(uploaded_);
 }
 
-int DeviceBatch::reserve(size_t ns, size_t nt) {
+// Grows the device tables geometrically; a regrowth frees the old tables
+// after the stream has finished with them.
+int DeviceBatch::reserve(size_t ns, size_t nt, hipStream_t st) {
+    if (ns > capSims_ || nt > capTiles_) KAM_HIP(hipStreamSynchronize(st));
     if (ns > capSims_) {
         if (dSims_) KAM_HIP(hipFree(dSims_));
         dSims_ = nullptr;
-        const size_t c = ns < 16 ? 16 : ns;
+        const size_t c = std::max<size_t>(std::max<size_t>(16, ns), 2 * capSims_);
         KAM_HIP(hipMalloc(&dSims_, c * sizeof(SimDesc)));
         capSims_ = c;
     }
     if (nt > capTiles_) {
         if (dTiles_) KAM_HIP(hipFree(dTiles_));
         dTiles_ = nullptr;
-        const size_t c = nt < 256 ? 256 : nt;
+        const size_t c = std::max<size_t>(std::max<size_t>(256, nt), 2 * capTiles_);
         KAM_HIP(hipMalloc(&dTiles_, c * sizeof(Tile)));
         capTiles_ = c;
     }
+    const size_t need = capSims_ * sizeof(SimDesc) + capTiles_ * sizeof(Tile);
+    if (need > stageBytes_) {
+        if (uploaded_) KAM_HIP(hipEventSynchronize(uploaded_));
+        if (hStage_) KAM_HIP(hipHostFree(hStage_));
+        hStage_ = nullptr;
+        KAM_HIP(hipHostMalloc(&hStage_, need, hipHostMallocDefault));
+        stageBytes_ = need;
+    }
+    if (!uploaded_) KAM_HIP(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming));
     return KAM_OK;
 }
 
@@ -300,15 +315,20 @@ int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>
         const int cnt = (int)hTiles_.size() - off;
         if (cnt > 0) tri_.push_back(TriClass{c[0], c[1], off, cnt});
     }
-    int rc = reserve(sims.size(), hTiles_.size());
+    int rc = reserve(sims.size(), hTiles_.size(), st);
     if (rc != KAM_OK) return rc;
-    if (!sims.empty())
-        KAM_HIP(hipMemcpyAsync(dSims_, sims.data(), sims.size() * sizeof(SimDesc),
+    // the staging buffer is free once the previous upload has landed (long
+    // ago, normally); the copies are queued behind the steps already launched
+    KAM_HIP(hipEventSynchronize(uploaded_));
+    char* stage = static_cast<char*>(hStage_);
+    const size_t sb = sims.size() * sizeof(SimDesc), tb = hTiles_.size() * sizeof(Tile);
+    std::memcpy(stage, sims.data(), sb);
+    std::memcpy(stage + capSims_ * sizeof(SimDesc), hTiles_.data(), tb);
+    if (sb) KAM_HIP(hipMemcpyAsync(dSims_, stage, sb, hipMemcpyHostToDevice, st));
+    if (tb)
+        KAM_HIP(hipMemcpyAsync(dTiles_, stage + capSims_ * sizeof(SimDesc), tb,
                                hipMemcpyHostToDevice, st));
-    if (!hTiles_.empty())
-        KAM_HIP(hipMemcpyAsync(dTiles_, hTiles_.data(), hTiles_.size() * sizeof(Tile),
-                               hipMemcpyHostToDevice, st));
-    KAM_HIP(hipStreamSynchronize(st));  // host vectors may change after return
+    KAM_HIP(hipEventRecord(uploaded_, st));
     nsims_ = (int)sims.size();
     return KAM_OK;
 }

@@ -9,6 +9,7 @@ ENGINE_SYMBOLS = [
     "kam_engine_create", "kam_engine_destroy", "kam_engine_admit", "kam_engine_step",
     "kam_engine_poll", "kam_engine_fetch", "kam_engine_cancel", "kam_engine_sync",
     "kam_engine_stats", "kam_engine_reset_stats", "kam_engine_profile", "kam_engine_now",
+    "kam_engine_completed",
 ]
 
 REQ_ACTIVE, REQ_DONE, REQ_CANCELLED, REQ_FAILED = 1, 2, 3, 4
@@ -17,7 +18,7 @@ KAM_EAGAIN = -4
 
 class EngineDesc(C.Structure):
     _fields_ = [("device", C.c_int), ("max_slots", C.c_int), ("hbm_budget_bytes", C.c_size_t),
-                ("use_graph", C.c_int), ("reserved", C.c_int)]
+                ("use_graph", C.c_int), ("max_inflight", C.c_int)]
 
 
 class ReqStatus(C.Structure):
@@ -51,17 +52,19 @@ def bind(L):
     L.kam_engine_profile.argtypes = [vp, C.c_int, C.POINTER(C.c_double)]
     L.kam_engine_now.restype = C.c_double
     L.kam_engine_now.argtypes = [vp]
+    L.kam_engine_completed.argtypes = [vp, C.POINTER(C.c_uint64), C.c_int, C.POINTER(C.c_int)]
 
 
 class Engine:
     """Batched / served stepping of many simulations on one GPU."""
 
-    def __init__(self, device=0, max_slots=0, hbm_budget_bytes=0, use_graph=True):
+    def __init__(self, device=0, max_slots=0, hbm_budget_bytes=0, use_graph=True, max_inflight=0):
         from . import check, lib
 
         self._lib = lib()
         self._check = check
-        d = EngineDesc(int(device), int(max_slots), int(hbm_budget_bytes), int(bool(use_graph)), 0)
+        d = EngineDesc(int(device), int(max_slots), int(hbm_budget_bytes), int(bool(use_graph)),
+                       int(max_inflight))
         h = C.c_void_p()
         check(self._lib.kam_engine_create(C.byref(d), C.byref(h)))
         self._h = h
@@ -118,6 +121,10 @@ class Engine:
         self._check(self._lib.kam_engine_fetch(self._h, C.c_uint64(rid), _ptr(a), _ptr(b), _ptr(c)))
         return a, b, c
 
+    def fetch_release(self, rid):
+        """Forgets a finished request without copying its fields."""
+        self._check(self._lib.kam_engine_fetch(self._h, C.c_uint64(rid), None, None, None))
+
     def cancel(self, rid):
         self._check(self._lib.kam_engine_cancel(self._h, C.c_uint64(rid)))
 
@@ -139,3 +146,10 @@ class Engine:
 
     def now(self):
         return self._lib.kam_engine_now(self._h)
+
+    def completed(self, max_ids=4096):
+        """Ids whose final fields reached the host since the last call."""
+        buf = (C.c_uint64 * max_ids)()
+        n = C.c_int(0)
+        self._check(self._lib.kam_engine_completed(self._h, buf, int(max_ids), C.byref(n)))
+        return [buf[k] for k in range(n.value)]

@@ -148,3 +148,33 @@ def test_engine_stats_latency():
         assert (k > 0).all()
         log(f"[engine] 4x256x512: p50 {s.step_p50_ms:.3f} ms p99 {s.step_p99_ms:.3f} ms "
             f"kernels {np.round(k, 4)}")
+
+
+def test_poisson_serving_stream_completes_and_matches_solo():
+    """A short Poisson stream through serve(): every request completes, the
+    completion query reports each once, and sampled results equal solo runs."""
+    from kaminogpu_amd import serve
+
+    reqs = serve.make_requests(24, seed=11, lo=32, hi=96, min_steps=5, max_steps=40)
+    pool = serve.StatePool(L=4, variants=2)
+    with K.Engine(max_slots=6) as eng:
+        out = serve.serve(reqs, rate=400.0, engine=eng, pool=pool)
+    assert out["requests"] == 24
+    assert all(r.done >= r.admit >= r.arrive for r in reqs)
+    assert out["request_p99_s"] >= out["request_p50_s"] > 0
+    log(f"[serve] 24 requests: {out['cell_steps_per_s']:.3e} cell-steps/s, request p50 "
+        f"{out['request_p50_s'] * 1e3:.1f} ms p99 {out['request_p99_s'] * 1e3:.1f} ms")
+    # the same requests again, with results kept, against solo runs
+    with K.Engine(max_slots=6) as eng:
+        ids = {}
+        for r in reqs[:6]:
+            ids[eng.admit(r.n_theta, 0.5 / r.n_theta, *pool.get(r.n_theta, r.seed), steps=r.steps)] = r
+        eng.step(max(r.steps for r in reqs[:6]))
+        eng.sync()
+        got = set(eng.completed())
+        assert got == set(ids)
+        for rid, r in ids.items():
+            res = eng.fetch(rid, r.n_theta)
+            ref = alone(r.n_theta, 0.5 / r.n_theta, pool.get(r.n_theta, r.seed), r.steps)
+            for g, x in zip(res, ref):
+                np.testing.assert_array_equal(g, x)
