@@ -46,8 +46,8 @@ def build_driver() -> str | None:
     if not os.path.exists(src):
         return None
     os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
-    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-o", DRIVER, src,
-          "-L", PKG, "-lkamino", "-Wl,-rpath,$ORIGIN/.."])
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), "-o", DRIVER, src,
+          "-L", PKG, "-lkamino", "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath-link,/opt/rocm/lib"])
     return DRIVER
 
 

@@ -1,0 +1,112 @@
+"""The configKamino.txt driver (kaminogpu_amd/bin/kamino): CPU checks of its
+config handling, and GPU checks that its frame schedule and outputs match the
+library run directly."""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import kaminogpu_amd as K
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "kaminogpu_amd", "bin", "kamino")
+
+
+def run(args, cwd):
+    return subprocess.run([DRIVER, *args], cwd=cwd, capture_output=True, text=True, timeout=120)
+
+
+def write_cfg(path, tokens):
+    with open(path, "w") as f:
+        f.write(" ".join(str(t) for t in tokens) + "\n")
+
+
+def frame_steps(dt, DT, frames):
+    """Steps per frame of Kamino::run (KaminoCore.cu:886-904), float32 clock."""
+    T = np.float32(0.0)
+    out = []
+    for i in range(1, frames + 1):
+        n = 0
+        while T < np.float32(i) * np.float32(DT):
+            n += 1
+            T = np.float32(T + np.float32(dt))
+        out.append(n + 1)
+        T = np.float32(np.float32(i) * np.float32(DT))
+    return out
+
+
+def test_config_errors_are_reported(tmp_path):
+    write_cfg(tmp_path / "short.txt", [1.0, 32, 200])
+    r = run([str(tmp_path / "short.txt")], tmp_path)
+    assert r.returncode == 1 and "16 tokens" in r.stderr
+    write_cfg(tmp_path / "bad.txt", [1.0, 2, 200, 0.005, 0.02, 1, 0, 1, 1, 1, 1, "a", "b", "null", "null", "null"])
+    r = run([str(tmp_path / "bad.txt")], tmp_path)
+    assert r.returncode == 1 and "nTheta" in r.stderr
+    r = run([], tmp_path)
+    assert r.returncode == 1 and "Usage" in r.stdout
+
+
+def test_no_device_is_an_error_not_a_crash(tmp_path):
+    if K.device_count() > 0:
+        pytest.skip("a device is present")
+    write_cfg(tmp_path / "c.txt", [1.0, 32, 200, 0.005, 0.02, 1, 0, 1, 1, 1, 1,
+                                   str(tmp_path / "f"), str(tmp_path / "p"), "null", "null", "null"])
+    r = run([str(tmp_path / "c.txt")], tmp_path)
+    assert r.returncode == 2 and "device" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_frames_match_library_run(tmp_path):
+    nt, dt, DT, frames = 32, 0.005, 0.0123, 3
+    write_cfg(tmp_path / "c.txt", [1.0, nt, 200, dt, DT, frames, 0, 1, 1, 1, 1,
+                                   str(tmp_path / "frame"), str(tmp_path / "part"), "null", "null",
+                                   "null", "seed=5", "L=6", "density=noise", "format=raw"])
+    r = run([str(tmp_path / "c.txt")], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "Frame 3 is ready" in r.stdout
+    steps = frame_steps(dt, DT, frames)
+    st = K.init_fields(nt, 6, 5, K.DENSITY_NOISE)
+    with K.Sim(nt, dt) as s:
+        s.set_fields(*st)
+        for i, n in enumerate(steps, start=1):
+            s.step(n)
+            ref = s.get_fields()
+            raw = open(tmp_path / f"frame{i}.kamr", "rb").read()
+            assert raw[:4] == b"KAMR"
+            a, b, k = struct.unpack("<iiq", raw[4:20])
+            assert (a, b, k) == (nt, 2 * nt, sum(steps[:i]))
+            data = np.frombuffer(raw[28:], np.float32)
+            N = nt * 2 * nt
+            np.testing.assert_array_equal(data[:N].reshape(nt, 2 * nt), ref[0])
+            np.testing.assert_array_equal(data[N:2 * N - 2 * nt].reshape(nt - 1, 2 * nt), ref[1])
+            np.testing.assert_array_equal(data[2 * N - 2 * nt:].reshape(nt, 2 * nt), ref[2])
+
+
+@pytest.mark.gpu
+def test_driver_bgeo_layout(tmp_path):
+    nt = 16
+    write_cfg(tmp_path / "c.txt", [2.0, nt, 200, 0.01, 0.02, 1, 0, 1, 1, 1, 1,
+                                   str(tmp_path / "frame"), str(tmp_path / "part"), "null", "null",
+                                   "null"])
+    r = run([str(tmp_path / "c.txt")], tmp_path)
+    assert r.returncode == 0, r.stderr
+    b = open(tmp_path / "frame1.bgeo", "rb").read()
+    assert b[:5] == b"BgeoV"
+    ver, npts, nprim, _, _, npattr = struct.unpack(">6i", b[5:29])
+    assert (ver, npts, nprim, npattr) == (5, nt * 2 * nt, 1, 2)
+    off = 29 + 12  # vertex / primitive / detail attribute counts
+    names = []
+    for _ in range(npattr):
+        (ln,) = struct.unpack(">H", b[off:off + 2])
+        name = b[off + 2:off + 2 + ln].decode()
+        size, _typ = struct.unpack(">Hi", b[off + 2 + ln:off + 8 + ln])
+        names.append((name, size))
+        off += 8 + ln + 4 * size
+    assert names == [("v", 3), ("density", 1)]
+    pts = np.frombuffer(b[off:off + npts * 8 * 4], ">f4").reshape(npts, 8)
+    radius = np.linalg.norm(pts[:, :3], axis=1)
+    np.testing.assert_allclose(radius, 2.0, rtol=1e-6)  # points on the sphere
+    np.testing.assert_allclose(pts[:, 3], 1.0)
+    assert b[-2:] == b"\x00\xff"
