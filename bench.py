@@ -171,6 +171,11 @@ def main():
         plain = cells * a.plain_steps / (p_total * 1e-3)
         log(f"plain: {plain:.3e} cell-steps/s, {p_total / a.plain_steps:.4f} ms/step")
 
+    # ---- tile shapes for this sphere size, timed on the device (kam_autotune)
+    tuned_ms = K.autotune(nt, min(B, 32), steps=5)
+    tiles = dict(zip(["adv_rows", "adv_cols", "fft_rows"], K.tune_get(nt)))
+    log(f"autotuned {nt}: {tiles}, kernel ms {np.round(tuned_ms, 4)}")
+
     # ---- batched form: B spheres through the engine, graph-replayed steps
     states = [init_state(nt, 16, 1 + b, 1) for b in range(min(B, 8))]
     total_steps = a.warmup + a.steps + a.profile_steps + 10
@@ -210,7 +215,8 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (seeded spherical-harmonic streamfunction, density band)",
         "config": {"workload": f"batched: {B} spheres of {nt}x{2 * nt}, dt {dt}, one engine step "
-                               f"advances all {B}", "n_theta": nt, "n_phi": 2 * nt, "sims": B},
+                               f"advances all {B}", "n_theta": nt, "n_phi": 2 * nt, "sims": B,
+                   "tiles": tiles},
         "batched": {"cell_steps_per_s": batched, "p50_step_ms": st.step_p50_ms,
                     "p99_step_ms": st.step_p99_ms, "speedup_vs_plain": batched / plain if plain else None},
         "plain": {"cell_steps_per_s": plain, "steps": a.plain_steps,

@@ -74,7 +74,7 @@ const char *kam_version(void);
 int kam_device_count(void);
 
 /* Creates one simulation with its own stream and device buffers (28 bytes per
- * cell).  Replaces KaminoSolver::KaminoSolver (KaminoSolver.cu:12-67). */
+ * cell, plus 256 bytes).  Replaces KaminoSolver::KaminoSolver (KaminoSolver.cu:12-67). */
 int kam_sim_create(const kam_sim_desc *desc, kam_sim **out);
 
 /* Frees the simulation.  Replaces KaminoSolver::~KaminoSolver
@@ -117,7 +117,11 @@ int kam_sim_benchmark(kam_sim *sim, int nsteps, int per_kernel, double *step_ms,
 /* Waits for all queued work of the simulation. */
 int kam_sim_sync(kam_sim *sim);
 
-/* Bytes of device memory one simulation of this shape needs. */
+/* Watchdog word after all queued steps: 0 = healthy, bit 0 set once a step
+ * produced a non-finite value (no counterpart in the reference). */
+int kam_sim_status(kam_sim *sim, int *status);
+
+/* Bytes of device memory one simulation of this shape needs (28 per cell + 256). */
 size_t kam_sim_bytes(int n_theta, int n_phi);
 
 /* Seeded synthetic initial state (host, FP32): the discrete curl of a real
@@ -151,6 +155,8 @@ typedef struct kam_engine_desc {
     int max_inflight;         /* steps queued ahead of the GPU at most (0 = 2) */
 } kam_engine_desc;
 
+/* KAM_REQ_FAILED: quarantined because a step produced a non-finite value in
+ * that simulation (checked every 8 steps); the others are unaffected. */
 enum { KAM_REQ_ACTIVE = 1, KAM_REQ_DONE = 2, KAM_REQ_CANCELLED = 3, KAM_REQ_FAILED = 4 };
 
 typedef struct kam_req_status {
@@ -173,6 +179,7 @@ typedef struct kam_stats {
     double step_max_ms;
     int64_t admitted, completed, rejected, cancelled;
     size_t bytes_in_use, bytes_budget;
+    int64_t failed;          /* requests quarantined by the non-finite watchdog */
 } kam_stats;
 
 int kam_engine_create(const kam_engine_desc *desc, kam_engine **out);
@@ -215,6 +222,28 @@ int kam_engine_profile(kam_engine *eng, int nsteps, double *kernel_ms);
 /* Host clock of the engine (s since creation), the time base of the request
  * timestamps. */
 double kam_engine_now(kam_engine *eng);
+
+/* ==== Autotuning (no counterpart in the reference, whose launch shape is
+ * fixed by determineLayout, KaminoSolver.cu:165-180) =======================
+ * Tile shapes per n_theta: advection rows x columns and FFT rows.  They
+ * regroup work only and never change a computed value. */
+
+/* Records tile shapes for n_theta (0 keeps a default); later plans use them. */
+int kam_tune_set(int n_theta, int adv_rows, int adv_cols, int fft_rows);
+
+/* The shapes a simulation of n_theta would get now. */
+int kam_tune_get(int n_theta, int *adv_rows, int *adv_cols, int *fft_rows);
+
+/* Times candidate shapes on `batch` synthetic simulations of n_theta (steps
+ * timed steps each), records the fastest with kam_tune_set, appends a line
+ * "gfx950 n_theta batch adv_rows adv_cols fft_rows adv_ms fft_ms" to
+ * cache_path (may be NULL), and returns the best per-kernel times in
+ * kernel_ms[4] (advection, div+FFT, tridiagonal, inverse FFT+gradient). */
+int kam_autotune(int device, int n_theta, int batch, int steps, const char *cache_path,
+                 double *kernel_ms);
+
+/* Loads gfx950 entries of a cache written by kam_autotune. */
+int kam_tune_load(const char *cache_path);
 
 #ifdef __cplusplus
 }

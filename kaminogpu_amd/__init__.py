@@ -28,8 +28,9 @@ PREC_FP32 = 0
 ABI_SYMBOLS = [
     "kam_last_error", "kam_version", "kam_device_count",
     "kam_sim_create", "kam_sim_destroy", "kam_sim_set_fields", "kam_sim_get_fields",
-    "kam_sim_step", "kam_sim_step_phases", "kam_sim_get_spectrum", "kam_sim_sync",
+    "kam_sim_step", "kam_sim_step_phases", "kam_sim_get_spectrum", "kam_sim_sync", "kam_sim_status",
     "kam_sim_bytes", "kam_sim_benchmark", "kam_init_fields", "kam_engine_completed",
+    "kam_tune_set", "kam_tune_get", "kam_autotune", "kam_tune_load",
     "kam_engine_create", "kam_engine_destroy", "kam_engine_admit", "kam_engine_step",
     "kam_engine_poll", "kam_engine_fetch", "kam_engine_cancel", "kam_engine_sync",
     "kam_engine_stats", "kam_engine_reset_stats", "kam_engine_profile", "kam_engine_now",
@@ -72,10 +73,16 @@ def lib():
     L.kam_sim_step_phases.argtypes = [C.c_void_p, C.c_int]
     L.kam_sim_get_spectrum.argtypes = [C.c_void_p, _fp]
     L.kam_sim_sync.argtypes = [C.c_void_p]
+    L.kam_sim_status.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     _dpp = C.POINTER(C.c_double)
     L.kam_sim_benchmark.argtypes = [C.c_void_p, C.c_int, C.c_int, _dpp, _dpp, _dpp]
     L.kam_sim_bytes.restype = C.c_size_t
     L.kam_sim_bytes.argtypes = [C.c_int, C.c_int]
+    L.kam_tune_set.argtypes = [C.c_int] * 4
+    _ip = C.POINTER(C.c_int)
+    L.kam_tune_get.argtypes = [C.c_int, _ip, _ip, _ip]
+    L.kam_autotune.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, _dpp]
+    L.kam_tune_load.argtypes = [C.c_char_p]
     L.kam_init_fields.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, C.c_int, _fp, _fp, _fp]
     _bind_engine(L)
     _lib = L
@@ -107,6 +114,25 @@ def _ptr(a):
 
 def device_count() -> int:
     return lib().kam_device_count()
+
+
+def tune_get(n_theta):
+    r, w, f = C.c_int(), C.c_int(), C.c_int()
+    check(lib().kam_tune_get(int(n_theta), C.byref(r), C.byref(w), C.byref(f)))
+    return r.value, w.value, f.value
+
+
+def tune_set(n_theta, adv_rows=0, adv_cols=0, fft_rows=0):
+    check(lib().kam_tune_set(int(n_theta), int(adv_rows), int(adv_cols), int(fft_rows)))
+
+
+def autotune(n_theta, batch, steps=10, cache_path=None, device=0):
+    """Times candidate tile shapes on the device and keeps the fastest;
+    returns the best per-kernel times (ms)."""
+    k = (C.c_double * 4)()
+    check(lib().kam_autotune(int(device), int(n_theta), int(batch), int(steps),
+                             cache_path.encode() if cache_path else None, k))
+    return np.array(list(k))
 
 
 DENSITY_NOISE, DENSITY_BAND = 0, 1
@@ -183,6 +209,12 @@ class Sim:
 
     def sync(self):
         check(lib().kam_sim_sync(self._h))
+
+    def status(self):
+        """Watchdog word: 0 healthy, bit 0 once a step produced a non-finite value."""
+        v = C.c_int(0)
+        check(lib().kam_sim_status(self._h, C.byref(v)))
+        return v.value
 
     def benchmark(self, nsteps, per_kernel=False):
         """Times nsteps steps with HIP events on the simulation's stream.

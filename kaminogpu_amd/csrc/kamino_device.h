@@ -8,7 +8,8 @@
 //   state[b][2] rho     [nTheta][nPhi]
 //   spec        [M][nTheta] float2          half spectrum, mode-major, M = nPhi/2;
 //                                           slot 0 packs (mode 0, Nyquist mode)
-// 28 bytes per cell in total.  Per-size tables (row metrics, FP64 Poisson
+// 28 bytes per cell in total (plus one 256-byte block for the watchdog word of
+// a stand-alone simulation; engine simulations keep theirs in a slot table).  Per-size tables (row metrics, FP64 Poisson
 // rows, twiddles) are shared by every simulation of the same shape and dt.
 #pragma once
 
@@ -57,6 +58,7 @@ struct SimDesc {
     float invNP;           // 1 / nPhi
     int advCols;           // columns per advection tile (W <= min(nPhi, 255))
     unsigned magicW[3];    // ceil(2^32 / (W + k)), k = 0, 1, 2
+    int* status;           // watchdog word: bit 0 set when a step produced a non-finite value
 };
 
 // One unit of work: rows [row0, row0 + rows) of simulation `sim` (or, for the

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -107,6 +108,8 @@ struct Request {
     SimDesc desc{};
     size_t off = 0, bytes = 0;
     int64_t total = 0, done = 0;
+    int slot = -1;            // index of its watchdog word
+    int64_t admitStep = 0;    // engine step count at admission
     int state = KAM_REQ_ACTIVE;
     bool copying = false;  // retired; final copy to the host still in flight
     double admit_s = 0, done_s = 0;
@@ -130,6 +133,14 @@ struct kam_engine {
     std::deque<uint64_t> draining;    // retired, final copy in flight (retirement order)
     std::vector<uint64_t> completed;  // final copy landed, not yet reported
     uint64_t nextId = 1;
+    // watchdog: one word per slot on the device, snapshots on the host
+    int* dStatus = nullptr;
+    int* hStatus = nullptr;
+    hipEvent_t statusEv = nullptr;
+    int64_t statusStep = -1;          // engine step of the pending snapshot (-1: none)
+    std::vector<int> freeSlots;
+    int64_t failed = 0;
+    int64_t totalSteps = 0;           // never reset
     int parity = 0;
     bool dirty = true;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -188,7 +199,19 @@ int rebuild(kam_engine* e) {
     });
     e->curCells = 0;
     for (uint64_t id : e->active) {
-        const Request& r = e->reqs[id];
+        Request& r = e->reqs[id];
+        // pick up tuning-table changes; tile shapes never change values
+        Plan fresh;
+        if (make_plan(r.plan.nT, r.plan.nP, fresh) == KAM_OK) {
+            r.plan = fresh;
+            r.desc.advRows = fresh.advRows;
+            r.desc.fftRows = fresh.fftRows;
+            r.desc.advCols = fresh.advCols;
+            for (int k = 0; k < 3; ++k) {
+                const uint64_t w = (uint64_t)fresh.advCols + k;
+                r.desc.magicW[k] = (unsigned)((((uint64_t)1 << 32) + w - 1) / w);
+            }
+        }
         sims.push_back(r.desc);
         plans.push_back(r.plan);
         e->curCells += (int64_t)r.plan.nT * r.plan.nP;
@@ -249,6 +272,8 @@ int retire(kam_engine* e, Request& r, int state) {
         e->draining.push_back(r.id);
     }
     r.state = state;
+    if (r.slot >= 0) e->freeSlots.push_back(r.slot);
+    r.slot = -1;
     e->arena.release(r.off, r.bytes);
     release_tables(e->ed.device, r.sd.n_theta, r.sd.dt, r.sd.radius);
     e->active.erase(std::find(e->active.begin(), e->active.end(), r.id));
@@ -269,6 +294,32 @@ void drain(kam_engine* e) {
         }
         e->draining.pop_front();
     }
+}
+
+// Every 8 steps: quarantine requests whose watchdog word was set in the last
+// snapshot, then take a new snapshot (copied on the stream, read 8 steps later).
+int watchdog(kam_engine* e) {
+    if (e->totalSteps % 8 != 0) return KAM_OK;
+    if (e->statusStep >= 0 && hipEventQuery(e->statusEv) == hipSuccess) {
+        std::vector<uint64_t> bad;
+        for (uint64_t id : e->active) {
+            const Request& r = e->reqs[id];
+            if (r.admitStep < e->statusStep && (e->hStatus[r.slot] & 1)) bad.push_back(id);
+        }
+        for (uint64_t id : bad) {
+            int rc = retire(e, e->reqs[id], KAM_REQ_FAILED);
+            if (rc != KAM_OK) return rc;
+            e->failed++;
+        }
+        e->statusStep = -1;
+    }
+    if (e->statusStep < 0 && !e->active.empty()) {
+        KAM_HIP(hipMemcpyAsync(e->hStatus, e->dStatus, (size_t)e->ed.max_slots * sizeof(int),
+                               hipMemcpyDeviceToHost, e->stream));
+        KAM_HIP(hipEventRecord(e->statusEv, e->stream));
+        e->statusStep = e->totalSteps;
+    }
+    return KAM_OK;
 }
 
 int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
@@ -320,6 +371,9 @@ int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
     }
     drain(e);
     reap_graphs(e, false);
+    e->totalSteps++;
+    int rc = watchdog(e);
+    if (rc != KAM_OK) return rc;
     if (e->stepEvents.size() > 1024) return collect(e, false);
     return KAM_OK;
 }
@@ -352,6 +406,14 @@ int kam_engine_create(const kam_engine_desc* d, kam_engine** out) {
         delete e;
         return rc != KAM_OK ? rc : set_err(KAM_EHIP, "stream creation failed");
     }
+    const size_t sb = (size_t)e->ed.max_slots * sizeof(int);
+    if (hipMalloc(&e->dStatus, sb) != hipSuccess || hipHostMalloc(&e->hStatus, sb, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&e->statusEv, hipEventDisableTiming) != hipSuccess ||
+        hipMemset(e->dStatus, 0, sb) != hipSuccess) {
+        kam_engine_destroy(e);
+        return set_err(KAM_EHIP, "watchdog table allocation failed");
+    }
+    for (int k = e->ed.max_slots - 1; k >= 0; --k) e->freeSlots.push_back(k);
     *out = e;
     return KAM_OK;
 }
@@ -371,6 +433,9 @@ int kam_engine_destroy(kam_engine* e) {
     for (hipEvent_t ev : e->stepEvents) (void)hipEventDestroy(ev);
     for (hipEvent_t ev : e->eventPool) (void)hipEventDestroy(ev);
     if (e->startEvent) (void)hipEventDestroy(e->startEvent);
+    if (e->statusEv) (void)hipEventDestroy(e->statusEv);
+    if (e->dStatus) (void)hipFree(e->dStatus);
+    if (e->hStatus) (void)hipHostFree(e->hStatus);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return KAM_OK;
@@ -406,8 +471,13 @@ int kam_engine_admit(kam_engine* e, const kam_sim_desc* d, const float* u_phi,
     r.sd = *d;
     r.total = steps;
     r.admit_s = e->now();
+    r.admitStep = e->totalSteps;
+    r.slot = e->freeSlots.back();
+    e->freeSlots.pop_back();
     float* mem = reinterpret_cast<float*>(e->arena.base() + r.off);
     fill_desc(r.desc, r.plan, r.tables, *d, mem);
+    r.desc.status = e->dStatus + r.slot;
+    KAM_HIP(hipMemsetAsync(r.desc.status, 0, sizeof(int), e->stream));
     const size_t N = (size_t)d->n_theta * d->n_phi;
     // both state copies' u_theta padding row and rho default to zero
     KAM_HIP(hipMemsetAsync(mem, 0, 6 * N * sizeof(float), e->stream));
@@ -478,6 +548,11 @@ int kam_engine_fetch(kam_engine* e, uint64_t id, float* u_phi, float* u_theta, f
     auto it = e->reqs.find(id);
     if (it == e->reqs.end()) return set_err(KAM_ENOTFOUND, "unknown request %llu", (unsigned long long)id);
     Request& r = it->second;
+    if (r.state == KAM_REQ_FAILED) {
+        e->reqs.erase(it);
+        return set_err(KAM_ESTATE, "request %llu was quarantined (non-finite state)",
+                       (unsigned long long)id);
+    }
     if (r.state != KAM_REQ_DONE) return set_err(KAM_ESTATE, "request %llu has not finished",
                                                  (unsigned long long)id);
     KAM_HIP(hipEventSynchronize(r.copied));
@@ -551,6 +626,7 @@ int kam_engine_stats(kam_engine* e, kam_stats* s) {
     s->completed = e->ncompleted;
     s->rejected = e->rejected;
     s->cancelled = e->cancelled;
+    s->failed = e->failed;
     s->bytes_in_use = e->arena.used();
     s->bytes_budget = e->arena.size();
     return KAM_OK;
@@ -571,10 +647,119 @@ int kam_engine_reset_stats(kam_engine* e) {
     e->stepMs.clear();
     e->steps = 0;
     e->cellSteps = 0;
-    e->admitted = e->ncompleted = e->rejected = e->cancelled = 0;
+    e->admitted = e->ncompleted = e->rejected = e->cancelled = e->failed = 0;
     return KAM_OK;
 }
 
 double kam_engine_now(kam_engine* e) { return e ? e->now() : 0.0; }
+
+int kam_tune_set(int n_theta, int adv_rows, int adv_cols, int fft_rows) {
+    if (n_theta < 4 || adv_rows < 0 || adv_cols < 0 || fft_rows < 0)
+        return set_err(KAM_EINVAL, "bad tuning entry");
+    return tune_set(n_theta, adv_rows, adv_cols, fft_rows);
+}
+
+int kam_tune_get(int n_theta, int* adv_rows, int* adv_cols, int* fft_rows) {
+    Plan p;
+    int rc = make_plan(n_theta, 2 * n_theta, p);
+    if (rc != KAM_OK) return rc;
+    if (adv_rows) *adv_rows = p.advRows;
+    if (adv_cols) *adv_cols = p.advCols;
+    if (fft_rows) *fft_rows = p.fftRows;
+    return KAM_OK;
+}
+
+int kam_autotune(int device, int n_theta, int batch, int steps, const char* cache_path, double* kernel_ms) {
+    if (n_theta < 4 || batch < 1 || steps < 1) return set_err(KAM_EINVAL, "bad autotune arguments");
+    Plan base;
+    int rc = make_plan(n_theta, 2 * n_theta, base);
+    if (rc != KAM_OK) return rc;
+    kam_engine_desc ed{};
+    ed.device = device;
+    ed.max_slots = batch;
+    ed.hbm_budget_bytes = (size_t)batch * (kam_sim_bytes(n_theta, 2 * n_theta) + 4096) + (1 << 20);
+    ed.use_graph = 0;
+    kam_engine* e = nullptr;
+    if ((rc = kam_engine_create(&ed, &e)) != KAM_OK) return rc;
+    const int nP = 2 * n_theta;
+    std::vector<float> up((size_t)n_theta * nP), ut((size_t)(n_theta - 1) * nP), rho((size_t)n_theta * nP);
+    kam_init_fields(n_theta, nP, 1.0, std::min(16, std::max(1, n_theta / 4)), 7, 1, up.data(), ut.data(), rho.data());
+    kam_sim_desc d{};
+    d.n_theta = n_theta;
+    d.n_phi = nP;
+    d.radius = 1.0;
+    d.dt = 0.5 / n_theta;
+    d.geometric = 1;
+    d.precision = KAM_PREC_FP32;
+    d.device = device;
+    uint64_t id;
+    for (int b = 0; b < batch && rc == KAM_OK; ++b)
+        rc = kam_engine_admit(e, &d, up.data(), ut.data(), rho.data(), (int64_t)1 << 40, &id);
+    // candidates: advection tiles, then FFT tile heights, each timed on its own kernels
+    const int rowsC[4] = {4, 8, 12, 16}, colsC[2] = {64, 128};
+    const int fftMax = base.fftRows;
+    const int fftC[3] = {fftMax, std::max(1, fftMax / 2), std::max(1, fftMax / 4)};
+    double best[KC_COUNT] = {1e30, 1e30, 1e30, 1e30};
+    int bestR = base.advRows, bestW = base.advCols, bestF = base.fftRows;
+    double k[KC_COUNT];
+    for (int ci = 0; ci < 2 && rc == KAM_OK; ++ci)
+        for (int ri = 0; ri < 4 && rc == KAM_OK; ++ri) {
+            if (colsC[ci] > nP) continue;
+            tune_set(n_theta, rowsC[ri], colsC[ci], bestF);
+            e->dirty = true;
+            if ((rc = kam_engine_profile(e, 2, k)) != KAM_OK) break;
+            if ((rc = kam_engine_profile(e, steps, k)) != KAM_OK) break;
+            if (k[KC_ADVECT] < best[KC_ADVECT]) {
+                best[KC_ADVECT] = k[KC_ADVECT];
+                bestR = rowsC[ri];
+                bestW = colsC[ci];
+            }
+        }
+    for (int fi = 0; fi < 3 && rc == KAM_OK; ++fi) {
+        if (fi > 0 && fftC[fi] == fftC[fi - 1]) continue;
+        tune_set(n_theta, bestR, bestW, fftC[fi]);
+        e->dirty = true;
+        if ((rc = kam_engine_profile(e, 2, k)) != KAM_OK) break;
+        if ((rc = kam_engine_profile(e, steps, k)) != KAM_OK) break;
+        const double f = k[KC_DIVFFT] + k[KC_IFFT];
+        if (f < best[KC_DIVFFT] + best[KC_IFFT]) {
+            best[KC_DIVFFT] = k[KC_DIVFFT];
+            best[KC_IFFT] = k[KC_IFFT];
+            best[KC_TRI] = k[KC_TRI];
+            bestF = fftC[fi];
+        }
+    }
+    kam_engine_destroy(e);
+    if (rc != KAM_OK) return rc;
+    tune_set(n_theta, bestR, bestW, bestF);
+    if (kernel_ms)
+        for (int c = 0; c < KC_COUNT; ++c) kernel_ms[c] = best[c];
+    if (cache_path && *cache_path) {
+        FILE* f = std::fopen(cache_path, "a");
+        if (!f) return set_err(KAM_EINVAL, "cannot append to %s", cache_path);
+        std::fprintf(f, "gfx950 %d %d %d %d %d %.5f %.5f\n", n_theta, batch, bestR, bestW, bestF,
+                     best[KC_ADVECT], best[KC_DIVFFT] + best[KC_IFFT]);
+        std::fclose(f);
+    }
+    return KAM_OK;
+}
+
+int kam_tune_load(const char* cache_path) {
+    if (!cache_path) return set_err(KAM_EINVAL, "null path");
+    FILE* f = std::fopen(cache_path, "r");
+    if (!f) return set_err(KAM_ENOTFOUND, "no tuning cache at %s", cache_path);
+    char arch[32];
+    int nT, b, R, W, F;
+    double t1, t2;
+    int n = 0;
+    while (std::fscanf(f, "%31s %d %d %d %d %d %lf %lf", arch, &nT, &b, &R, &W, &F, &t1, &t2) == 8) {
+        if (std::strcmp(arch, "gfx950") == 0 && nT >= 4) {
+            tune_set(nT, R, W, F);  // later lines win
+            ++n;
+        }
+    }
+    std::fclose(f);
+    return n > 0 ? KAM_OK : set_err(KAM_ENOTFOUND, "no gfx950 entries in %s", cache_path);
+}
 
 }  // extern "C"

@@ -113,6 +113,12 @@ __device__ __forceinline__ float sample(Rsrc f, const Grid& g, int bi, float fph
     return lerpf(lerpf(a0, a1, ap), lerpf(b0, b1, ap), w);
 }
 
+// Watchdog: any non-finite result of a step marks the simulation (a vector
+// atomic; taken only on the rare failing path).
+__device__ __forceinline__ void flag_bad(int* status, bool bad) {
+    if (bad && status) atomicOr(status, 1);
+}
+
 // Node offsets of each grid in cells (KaminoHeader.cuh:30-41) and the shift
 // that maps a raw coordinate onto that grid's sample lattice.
 __device__ constexpr float kNodeP[3] = {-0.5f, 0.f, 0.f};
@@ -253,7 +259,9 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
                 const int i = c0 + c, j = j0 + r, o = j * nP + i;
                 oup[o] = advect_node<K_VPHI, K_VPHI>(up, ut, up, g, i, j, cofPC[j], cofT);
                 if (j < nT - 1) out[o] = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, g, i, j, cofPT[j], cofT);
-                orh[o] = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, g, i, j, cofPC[j], cofT);
+                const float rr = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, g, i, j, cofPC[j], cofT);
+                orh[o] = rr;
+                flag_bad(s.status, !isfinite(rr));
             }
             continue;
         }
@@ -307,6 +315,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
             const float b = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, g, i2, j2, cofPC[j2], cofT);
             orh[j * nP + i] = a;
             orh[j2 * nP + i2] = b;
+            flag_bad(s.status, !isfinite(a) || !isfinite(b));
         }
         __syncthreads();
 
@@ -760,8 +769,15 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
             const int il = (i == 0) ? nP - 1 : i - 1;
             const float pc = p[(size_t)r * nP + i];
             const size_t o = (size_t)j * nP + i;
-            up[o] -= (pc - p[(size_t)r * nP + il]) * invHs[j];
-            if (j < nT - 1) ut[o] -= (p[(size_t)(r + 1) * nP + i] - pc) * invH;
+            const float nu = up[o] - (pc - p[(size_t)r * nP + il]) * invHs[j];
+            up[o] = nu;
+            bool bad = !isfinite(nu);
+            if (j < nT - 1) {
+                const float nv = ut[o] - (p[(size_t)(r + 1) * nP + i] - pc) * invH;
+                ut[o] = nv;
+                bad |= !isfinite(nv);
+            }
+            flag_bad(s.status, bad);
         }
         __syncthreads();
     }

@@ -41,6 +41,8 @@ struct Plan {
 int make_plan(int nTheta, int nPhi, Plan& plan);
 // Overrides the tile heights (0 keeps the default); used by the autotuner.
 int adjust_plan(Plan& plan, int advRows, int fftRows);
+// Records tile shapes for n_theta (0 = default); later plans use them.
+int tune_set(int nTheta, int advRows, int advCols, int fftRows);
 
 // Device tables shared by every simulation of one (device, shape, dt, R).
 struct Tables {

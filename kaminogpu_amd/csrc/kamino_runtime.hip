@@ -56,6 +56,28 @@ static size_t adv_lds(int R, int W) {
 }
 static size_t fft_lds(int R, int M) { return (size_t)(R + 1) * M * 2 * sizeof(float); }
 
+// ---------------------------------------------------------------------------
+// Tuning table: tile shapes per n_theta, set by the autotuner (or by hand).
+// Tile shapes regroup work only; they never change a computed value.
+// ---------------------------------------------------------------------------
+struct TuneEntry {
+    int advRows = 0, advCols = 0, fftRows = 0;
+};
+static std::mutex g_tune_mu;
+static std::map<int, TuneEntry> g_tune;
+
+int tune_set(int nT, int advRows, int advCols, int fftRows) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[nT] = TuneEntry{advRows, advCols, fftRows};
+    return KAM_OK;
+}
+
+static TuneEntry tune_get(int nT) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(nT);
+    return it == g_tune.end() ? TuneEntry{} : it->second;
+}
+
 int make_plan(int nT, int nP, Plan& p) {
     if (nT < 4) return set_err(KAM_EINVAL, "n_theta must be >= 4 (got %d)", nT);
     if (nP != 2 * nT) return set_err(KAM_EINVAL, "n_phi must equal 2*n_theta (got %d, %d)", nP, nT);
@@ -83,7 +105,10 @@ int make_plan(int nT, int nP, Plan& p) {
     else if (nT <= 256) { p.triL = 32; p.triK = 8; }
     else if (nT <= 512) { p.triL = 64; p.triK = 8; }
     else { p.triL = 64; p.triK = 16; }
-    // manual overrides for tile sweeps
+    // tuned shapes, then manual overrides for tile sweeps
+    const TuneEntry te = tune_get(nT);
+    if (te.advCols >= 8 && te.advCols <= 254) p.advCols = te.advCols < nP ? te.advCols : nP;
+    if (te.advRows > 0 || te.fftRows > 0) adjust_plan(p, te.advRows, te.fftRows);
     if (const char* v = std::getenv("KAM_ADV_COLS")) {
         const int w = std::atoi(v);
         if (w >= 8 && w <= 254) p.advCols = w < nP ? w : nP;
@@ -109,7 +134,7 @@ int adjust_plan(Plan& p, int advRows, int fftRows) {
     return KAM_OK;
 }
 
-size_t sim_floats(int nT, int nP) { return (size_t)7 * nT * nP; }
+size_t sim_floats(int nT, int nP) { return (size_t)7 * nT * nP + 64; }
 
 // ---------------------------------------------------------------------------
 // Per-shape tables
@@ -192,6 +217,7 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
     for (int b = 0; b < 2; ++b)
         for (int f = 0; f < 3; ++f) d.state[b][f] = mem + (size_t)(3 * b + f) * N;
     d.spec = mem + 6 * N;
+    d.status = reinterpret_cast<int*>(mem + 7 * N);  // callers may point it elsewhere
     d.rowf = t.rowf;
     d.rowd = t.rowd;
     d.twN = t.twN;
@@ -607,6 +633,14 @@ int kam_sim_benchmark(kam_sim* s, int nsteps, int per_kernel, double* step_ms, d
     for (auto& e : ev) (void)hipEventDestroy(e);
     for (auto& e : kev) (void)hipEventDestroy(e);
     return rc;
+}
+
+int kam_sim_status(kam_sim* s, int* status) {
+    if (!s || !status) return set_err(KAM_EINVAL, "null argument");
+    KAM_HIP(hipSetDevice(s->sd.device));
+    KAM_HIP(hipMemcpyAsync(status, s->desc.status, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    KAM_HIP(hipStreamSynchronize(s->stream));
+    return KAM_OK;
 }
 
 int kam_sim_sync(kam_sim* s) {

@@ -31,7 +31,7 @@ class EngineStats(C.Structure):
                 ("cell_steps", C.c_int64), ("gpu_ms", C.c_double), ("step_p50_ms", C.c_double),
                 ("step_p99_ms", C.c_double), ("step_max_ms", C.c_double), ("admitted", C.c_int64),
                 ("completed", C.c_int64), ("rejected", C.c_int64), ("cancelled", C.c_int64),
-                ("bytes_in_use", C.c_size_t), ("bytes_budget", C.c_size_t)]
+                ("bytes_in_use", C.c_size_t), ("bytes_budget", C.c_size_t), ("failed", C.c_int64)]
 
 
 def bind(L):

@@ -38,8 +38,8 @@ def test_bad_shapes_are_rejected_without_a_gpu():
         assert L.kam_last_error().decode()
 
 
-def test_sim_bytes_is_28_per_cell():
-    assert K.lib().kam_sim_bytes(256, 512) == 28 * 256 * 512
+def test_sim_bytes_is_28_per_cell_plus_watchdog_block():
+    assert K.lib().kam_sim_bytes(256, 512) == 28 * 256 * 512 + 256
 
 
 def test_no_device_in_this_container_is_reported_not_fatal():
@@ -63,3 +63,13 @@ def test_product_initial_state_matches_the_oracle_generator():
         for g, r in zip(got, ref):
             assert g.dtype == np.float32
             np.testing.assert_allclose(g, r, rtol=2e-6, atol=1e-6 * max(1.0, np.abs(r).max()))
+
+
+def test_tuning_table_round_trip_without_a_gpu():
+    default = K.tune_get(192)
+    K.tune_set(192, 12, 64, 4)
+    assert K.tune_get(192) == (12, 64, 4)
+    K.tune_set(192, 0, 0, 0)
+    assert K.tune_get(192) == default
+    with pytest.raises(K.KaminoError):
+        K.tune_set(2, 1, 1, 1)

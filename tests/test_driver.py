@@ -68,7 +68,8 @@ def test_driver_frames_match_library_run(tmp_path):
     assert "Frame 3 is ready" in r.stdout
     steps = frame_steps(dt, DT, frames)
     st = K.init_fields(nt, 6, 5, K.DENSITY_NOISE)
-    with K.Sim(nt, dt) as s:
+    # the config's dt is read as a C float, as the reference's main.cu:12 does
+    with K.Sim(nt, float(np.float32(dt))) as s:
         s.set_fields(*st)
         for i, n in enumerate(steps, start=1):
             s.step(n)

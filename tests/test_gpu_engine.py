@@ -178,3 +178,61 @@ def test_poisson_serving_stream_completes_and_matches_solo():
             ref = alone(r.n_theta, 0.5 / r.n_theta, pool.get(r.n_theta, r.seed), r.steps)
             for g, x in zip(res, ref):
                 np.testing.assert_array_equal(g, x)
+
+
+def test_nan_quarantines_only_that_simulation():
+    """A non-finite value injected into one simulation's state sets its
+    watchdog word; the engine retires it as FAILED within 8 steps and every
+    other simulation runs on, equal to its solo run."""
+    reqs = [make_request(nt, 60 + k) for k, nt in enumerate([64, 48, 64])]
+    poisoned = [np.array(x, copy=True) for x in reqs[1]["state"]]
+    poisoned[0][10, 7] = np.nan
+    with K.Engine() as eng:
+        ids = [eng.admit(r["nt"], r["dt"], *(poisoned if k == 1 else r["state"]), steps=30)
+               for k, r in enumerate(reqs)]
+        eng.step(30)
+        eng.sync()
+        assert eng.poll(ids[1]).state == K.engine.REQ_FAILED
+        assert eng.poll(ids[1]).steps_done <= 17
+        for k in (0, 2):
+            assert eng.poll(ids[k]).state == K.engine.REQ_DONE
+            got = eng.fetch(ids[k], reqs[k]["nt"])
+            ref = alone(reqs[k]["nt"], reqs[k]["dt"], reqs[k]["state"], 30)
+            for g, x in zip(got, ref):
+                np.testing.assert_array_equal(g, x)
+        assert eng.stats().failed == 1
+        with pytest.raises(K.KaminoError):
+            eng.fetch(ids[1], reqs[1]["nt"])
+    with K.Sim(64, 0.01) as s:
+        st = [np.array(x, copy=True) for x in reqs[0]["state"]]
+        s.set_fields(*st)
+        s.step(2)
+        assert s.status() == 0
+        st[2][3, 3] = np.inf
+        s.set_fields(*st)
+        s.step(1)
+        assert s.status() & 1
+
+
+def test_autotune_runs_and_tile_shapes_never_change_values(tmp_path):
+    r = make_request(96, 5)
+    base = alone(96, r["dt"], r["state"], 4)
+    cache = str(tmp_path / "tune.txt")
+    k = K.autotune(96, batch=4, steps=2, cache_path=cache)
+    assert (k > 0).all()
+    assert open(cache).read().startswith("gfx950 96 4 ")
+    tuned = K.tune_get(96)
+    try:
+        for shape in [(4, 64, 2), (16, 96, 1), tuned]:
+            K.tune_set(96, *shape)
+            got = alone(96, r["dt"], r["state"], 4)
+            with K.Engine() as eng:
+                i = eng.admit(96, r["dt"], *r["state"], steps=4)
+                eng.step(4)
+                got_e = eng.fetch(i, 96)
+            for a, b, c in zip(got, got_e, base):
+                np.testing.assert_array_equal(a, c)
+                np.testing.assert_array_equal(b, c)
+    finally:
+        K.tune_set(96, 0, 0, 0)
+    log(f"[tune] 96: tuned {tuned}, best kernel ms {np.round(k, 4)}")
