@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven-steps", type=int, default=200, help="config-3 batch steps (0 skips)")
     ap.add_argument("--serve-requests", type=int, default=2000, help="config-4 requests (0 skips)")
+    ap.add_argument("--tune-cache", default=None,
+                    help="tuning cache: sizes it holds are loaded, the others autotuned and appended")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (for HBM counter scaling)")
     return ap.parse_args()
@@ -77,11 +79,32 @@ def cpu_baseline(nt, dt, seconds):
     }
 
 
-def run_uneven(K, steps, sims=64, seed=3):
+TUNED = set()
+
+
+def tune(K, nt, batch, cache):
+    """Tile shapes for n_theta: from the cache when it holds them, else timed on
+    the device (kam_autotune) and appended to the cache.  Returns the tiles and
+    the best per-kernel times (None when loaded)."""
+    if cache and os.path.exists(cache) and not TUNED:
+        TUNED.update(K.tune_load(cache))
+    ms = None
+    if nt not in TUNED:
+        ms = K.autotune(nt, batch, steps=5, cache_path=cache)
+        TUNED.add(nt)
+    return dict(zip(["adv_rows", "adv_cols", "fft_rows"], K.tune_get(nt))), ms
+
+
+def run_uneven(K, steps, cache, sims=64, seed=3):
     """Config 3: 64 spheres with n_theta drawn from {32,...,256}, dt = 0.5/n_theta,
-    L = min(16, n_theta/4), density band, stepped together as graph replays."""
+    L = min(16, n_theta/4), density band, stepped together as graph replays;
+    tiles of each size from the autotuner."""
     rng = np.random.default_rng(seed)
     sizes = rng.choice([32, 48, 64, 96, 128, 192, 256], sims)
+    tiles = {}
+    for nt in sorted(set(int(n) for n in sizes)):
+        tiles[nt], _ = tune(K, nt, max(1, int((sizes == nt).sum())), cache)
+    log(f"uneven tiles: {tiles}")
     with K.Engine(use_graph=True) as eng:
         for k, nt in enumerate(sizes):
             st = K.init_fields(int(nt), min(16, int(nt) // 4), 100 + k, K.DENSITY_BAND)
@@ -90,26 +113,37 @@ def run_uneven(K, steps, sims=64, seed=3):
         eng.reset_stats()
         eng.step(steps)
         s = eng.stats()
+        kern = eng.profile(10)
     v = s.cell_steps / (s.gpu_ms * 1e-3)
-    log(f"uneven x{sims}: {v:.3e} cell-steps/s, p50 {s.step_p50_ms:.4f} p99 {s.step_p99_ms:.4f} ms")
+    log(f"uneven x{sims}: {v:.3e} cell-steps/s, p50 {s.step_p50_ms:.4f} p99 {s.step_p99_ms:.4f} ms, "
+        f"kernel ms {np.round(kern, 4)}")
     return {"sims": sims, "cells": int(sum(2 * int(n) ** 2 for n in sizes)), "steps": steps,
-            "cell_steps_per_s": v, "p50_step_ms": s.step_p50_ms, "p99_step_ms": s.step_p99_ms}
+            "cell_steps_per_s": v, "p50_step_ms": s.step_p50_ms, "p99_step_ms": s.step_p99_ms,
+            "kernel_ms": dict(zip(["advect", "div_fft", "tridiag", "ifft_grad"], kern.tolist())),
+            "tiles": {str(k): t for k, t in tiles.items()}}
 
 
-def run_served(K, n, batched_rate):
-    """Config 4: Poisson arrivals at ~80% of the batched capacity; n_theta
-    log-uniform in [32, 512] (rounded to supported sizes), steps U{50..2000}."""
+def run_served(K, n):
+    """Config 4: Poisson arrivals at 80% of the mix's saturated capacity (128
+    requests of the same distribution stepped together); n_theta log-uniform
+    in [32, 512] (rounded to supported sizes), steps U{50..2000}."""
     from kaminogpu_amd import serve
 
     reqs = serve.make_requests(n, seed=4)
     work = float(np.mean([2 * r.n_theta ** 2 * r.steps for r in reqs]))
-    rate = 0.8 * batched_rate / work
+    pool = serve.StatePool()
     with K.Engine(use_graph=True) as eng:
-        out = serve.serve(reqs, rate, eng, log=log)
-    log(f"served {n}: {out['cell_steps_per_s']:.3e} cell-steps/s, step p50 {out['step_p50_ms']:.3f} "
-        f"p99 {out['step_p99_ms']:.3f} ms, request p50 {out['request_p50_s']:.3f} p99 "
-        f"{out['request_p99_s']:.3f} s")
+        cap = serve.capacity(serve.make_requests(128, seed=5), eng, pool)
+        rate = 0.8 * cap / work
+        log(f"serve: mix capacity {cap:.3e} cell-steps/s -> {rate:.1f} requests/s")
+        out = serve.serve(reqs, rate, eng, pool, log=log)
+    out["load"] = out["cell_steps_per_s"] / cap
+    log(f"served {n}: {out['cell_steps_per_s']:.3e} cell-steps/s ({out['load']:.2f} of capacity), "
+        f"GPU in steps {out['gpu_busy']:.2f} of the wall time, "
+        f"step p50 {out['step_p50_ms']:.3f} p99 {out['step_p99_ms']:.3f} ms, request p50 "
+        f"{out['request_p50_s']:.3f} p99 {out['request_p99_s']:.3f} s")
     out["mean_cell_steps_per_request"] = work
+    out["mix_capacity_cell_steps_per_s"] = cap
     return out
 
 
@@ -172,9 +206,8 @@ def main():
         log(f"plain: {plain:.3e} cell-steps/s, {p_total / a.plain_steps:.4f} ms/step")
 
     # ---- tile shapes for this sphere size, timed on the device (kam_autotune)
-    tuned_ms = K.autotune(nt, min(B, 32), steps=5)
-    tiles = dict(zip(["adv_rows", "adv_cols", "fft_rows"], K.tune_get(nt)))
-    log(f"autotuned {nt}: {tiles}, kernel ms {np.round(tuned_ms, 4)}")
+    tiles, tuned_ms = tune(K, nt, min(B, 32), a.tune_cache)
+    log(f"tiles {nt}: {tiles}" + ("" if tuned_ms is None else f", autotuned, kernel ms {np.round(tuned_ms, 4)}"))
 
     # ---- batched form: B spheres through the engine, graph-replayed steps
     states = [init_state(nt, 16, 1 + b, 1) for b in range(min(B, 8))]
@@ -193,8 +226,8 @@ def main():
     log(f"batched x{B}: {batched:.3e} cell-steps/s, {ms_per_step:.4f} ms/step, "
         f"p50 {st.step_p50_ms:.4f} p99 {st.step_p99_ms:.4f}")
 
-    uneven = run_uneven(K, a.uneven_steps) if a.uneven_steps > 0 else None
-    served = run_served(K, a.serve_requests, batched) if a.serve_requests > 0 else None
+    uneven = run_uneven(K, a.uneven_steps, a.tune_cache) if a.uneven_steps > 0 else None
+    served = run_served(K, a.serve_requests) if a.serve_requests > 0 else None
 
     kern = {k: float(v) for k, v in zip(KERNELS, kern_ms)}
     dom = max(kern, key=kern.get)

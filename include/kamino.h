@@ -121,6 +121,41 @@ int kam_sim_sync(kam_sim *sim);
  * produced a non-finite value (no counterpart in the reference). */
 int kam_sim_status(kam_sim *sim, int *status);
 
+/* The reference's default initial velocity (KaminoSolver::initialize_velocity,
+ * KaminoInitializer.cu:3-133, called from the solver's constructor at
+ * KaminoSolver.cu:59): the curl of four-octave FBM value noise, gain
+ * 4096 / n_phi, with its quirks kept (the separate phi = 0 seam loop :36-55;
+ * ll_y = ul_y in the u_theta loop, :69).  Host, FP32, computed in the
+ * reference's FP32/FP64 mix (the noise hashes FP32 coordinates).  Either
+ * output may be NULL. */
+int kam_init_fbm(int n_theta, int n_phi, double radius, float *u_phi, float *u_theta);
+
+/* Particles (KaminoParticles.cu, advectionParticles KaminoCore.cu:321-342).
+ * Sets n particles as (phi, theta) pairs in radians (host, FP32, 2n floats,
+ * the layout of KaminoParticles::coordCPUBuffer); every later step moves them
+ * one forward-Euler step with the pre-step velocities, inside the advection
+ * phase.  n = 0 removes them.  Needs 16 bytes of device memory per particle.
+ * Replaces KaminoParticles::KaminoParticles's upload (KaminoParticles.cu:3-83,
+ * copy2GPU :94-101). */
+int kam_sim_set_particles(kam_sim *sim, const float *coords, int64_t n);
+
+/* Downloads the n = current particle count (phi, theta) pairs after all
+ * queued steps.  Replaces KaminoParticles::copyBack2CPU
+ * (KaminoParticles.cu:84-91). */
+int kam_sim_get_particles(kam_sim *sim, float *coords, int64_t n);
+
+/* Particle count of the simulation (0 if none). */
+int64_t kam_sim_particle_count(const kam_sim *sim);
+
+/* Seeded particle layout of KaminoParticles.cu:20-80: sqrt(density) * n_theta
+ * particles per meridian and twice that per latitude, particle (i, j) at
+ * (i delta, j delta) + a jitter of up to delta / 2 in each coordinate with a
+ * random sign, negative coordinates clamped to 0, stored at i * n_j + j.
+ * The reference draws with C rand(), whose sequence is platform-defined; this
+ * draws the same four numbers per particle from SplitMix64(seed).  coords may
+ * be NULL to query *count. */
+int kam_particles_seed(int n_theta, double density, uint64_t seed, float *coords, int64_t *count);
+
 /* Bytes of device memory one simulation of this shape needs (28 per cell + 256). */
 size_t kam_sim_bytes(int n_theta, int n_phi);
 

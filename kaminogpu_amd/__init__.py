@@ -30,6 +30,8 @@ ABI_SYMBOLS = [
     "kam_sim_create", "kam_sim_destroy", "kam_sim_set_fields", "kam_sim_get_fields",
     "kam_sim_step", "kam_sim_step_phases", "kam_sim_get_spectrum", "kam_sim_sync", "kam_sim_status",
     "kam_sim_bytes", "kam_sim_benchmark", "kam_init_fields", "kam_engine_completed",
+    "kam_sim_set_particles", "kam_sim_get_particles", "kam_sim_particle_count", "kam_particles_seed",
+    "kam_init_fbm",
     "kam_tune_set", "kam_tune_get", "kam_autotune", "kam_tune_load",
     "kam_engine_create", "kam_engine_destroy", "kam_engine_admit", "kam_engine_step",
     "kam_engine_poll", "kam_engine_fetch", "kam_engine_cancel", "kam_engine_sync",
@@ -84,6 +86,12 @@ def lib():
     L.kam_autotune.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, _dpp]
     L.kam_tune_load.argtypes = [C.c_char_p]
     L.kam_init_fields.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, C.c_int, _fp, _fp, _fp]
+    L.kam_sim_set_particles.argtypes = [C.c_void_p, _fp, C.c_int64]
+    L.kam_sim_get_particles.argtypes = [C.c_void_p, _fp, C.c_int64]
+    L.kam_sim_particle_count.restype = C.c_int64
+    L.kam_sim_particle_count.argtypes = [C.c_void_p]
+    L.kam_particles_seed.argtypes = [C.c_int, C.c_double, C.c_uint64, _fp, C.POINTER(C.c_int64)]
+    L.kam_init_fbm.argtypes = [C.c_int, C.c_int, C.c_double, _fp, _fp]
     _bind_engine(L)
     _lib = L
     return L
@@ -135,7 +143,37 @@ def autotune(n_theta, batch, steps=10, cache_path=None, device=0):
     return np.array(list(k))
 
 
+def tune_load(cache_path):
+    """Loads a tuning cache written by autotune; returns the n_theta values it holds."""
+    check(lib().kam_tune_load(cache_path.encode()))
+    sizes = set()
+    with open(cache_path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 8 and parts[0] == "gfx950":
+                sizes.add(int(parts[1]))
+    return sizes
+
+
 DENSITY_NOISE, DENSITY_BAND = 0, 1
+
+
+def init_fbm(n_theta, radius=1.0):
+    """The reference's FBM curl-noise velocity (kam_init_fbm): FP32 (u_phi, u_theta)."""
+    nt, nph = int(n_theta), 2 * int(n_theta)
+    up = np.empty((nt, nph), np.float32)
+    ut = np.empty((nt - 1, nph), np.float32)
+    check(lib().kam_init_fbm(nt, nph, float(radius), _ptr(up), _ptr(ut)))
+    return up, ut
+
+
+def particles_seed(n_theta, density, seed):
+    """Jittered particle lattice (kam_particles_seed): float32 (n, 2) (phi, theta)."""
+    n = C.c_int64(0)
+    check(lib().kam_particles_seed(int(n_theta), float(density), int(seed), None, C.byref(n)))
+    c = np.empty((n.value, 2), np.float32)
+    check(lib().kam_particles_seed(int(n_theta), float(density), int(seed), _ptr(c), C.byref(n)))
+    return c
 
 
 def init_fields(n_theta, L, seed, density=DENSITY_BAND, radius=1.0):
@@ -206,6 +244,17 @@ class Sim:
         s = np.empty((M, self.n_theta, 2), np.float32)
         check(lib().kam_sim_get_spectrum(self._h, _ptr(s)))
         return s[..., 0] + 1j * s[..., 1]
+
+    def set_particles(self, coords):
+        """Particles as (n, 2) (phi, theta) radians; later steps move them."""
+        c = np.ascontiguousarray(coords, dtype=np.float32).reshape(-1, 2)
+        check(lib().kam_sim_set_particles(self._h, _ptr(c) if len(c) else None, len(c)))
+
+    def get_particles(self):
+        n = lib().kam_sim_particle_count(self._h)
+        c = np.empty((n, 2), np.float32)
+        check(lib().kam_sim_get_particles(self._h, _ptr(c) if n else None, n))
+        return c
 
     def sync(self):
         check(lib().kam_sim_sync(self._h))

@@ -9,7 +9,9 @@
 //   spec        [M][nTheta] float2          half spectrum, mode-major, M = nPhi/2;
 //                                           slot 0 packs (mode 0, Nyquist mode)
 // 28 bytes per cell in total (plus one 256-byte block for the watchdog word of
-// a stand-alone simulation; engine simulations keep theirs in a slot table).  Per-size tables (row metrics, FP64 Poisson
+// a stand-alone simulation; engine simulations keep theirs in a slot table).
+// Particles, when a simulation has them, live in their own allocation:
+//   part[b] float2 (phi, theta) [nPart], b = 0, 1 (ping-pong with the fields).  Per-size tables (row metrics, FP64 Poisson
 // rows, twiddles) are shared by every simulation of the same shape and dt.
 #pragma once
 
@@ -59,6 +61,9 @@ struct SimDesc {
     int advCols;           // columns per advection tile (W <= min(nPhi, 255))
     unsigned magicW[3];    // ceil(2^32 / (W + k)), k = 0, 1, 2
     int* status;           // watchdog word: bit 0 set when a step produced a non-finite value
+    float* part[2];        // particles, float2 (phi, theta) radians [nPart], per parity
+    int nPart;             // particle count (0: none)
+    float radius, dt;      // FP32, as the reference's constants (KaminoCore.cu:868-872)
 };
 
 // One unit of work: rows [row0, row0 + rows) of simulation `sim` (or, for the

@@ -147,7 +147,13 @@ struct kam_engine {
     // replaced graphs wait here until their last launch has finished
     std::deque<std::pair<hipGraphExec_t, hipEvent_t>> oldGraphs;
     // statistics
-    std::deque<hipEvent_t> stepEvents;  // recorded after each step, not yet collected
+    // Per step, not yet collected: the event recorded after it and, when the
+    // GPU had gone idle before the step was queued, one recorded just before
+    // it (else the step's interval starts at the previous step's end).
+    struct StepEv {
+        hipEvent_t start, end;
+    };
+    std::deque<StepEv> stepEvents;
     std::vector<hipEvent_t> eventPool;
     hipEvent_t startEvent = nullptr;    // marks the beginning of the stats window
     bool windowOpen = false;
@@ -240,13 +246,15 @@ hipEvent_t take_event(kam_engine* e) {
 int collect(kam_engine* e, bool wait) {
     if (wait) KAM_HIP(hipStreamSynchronize(e->stream));
     while (!e->stepEvents.empty()) {
-        hipEvent_t ev = e->stepEvents.front();
+        const kam_engine::StepEv se = e->stepEvents.front();
+        hipEvent_t ev = se.end;
         if (!wait && hipEventQuery(ev) != hipSuccess) break;
-        if (e->windowOpen) {
+        if (se.start || e->windowOpen) {
             float ms = 0.f;
-            KAM_HIP(hipEventElapsedTime(&ms, e->startEvent, ev));
+            KAM_HIP(hipEventElapsedTime(&ms, se.start ? se.start : e->startEvent, ev));
             e->stepMs.push_back(ms);
         }
+        if (se.start) e->eventPool.push_back(se.start);
         // the event becomes the start of the next interval
         if (e->startEvent) e->eventPool.push_back(e->startEvent);
         e->startEvent = ev;
@@ -325,12 +333,19 @@ int watchdog(kam_engine* e) {
 int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
     // keep at most max_inflight steps queued ahead of the GPU
     const size_t q = e->stepEvents.size(), lim = (size_t)e->ed.max_inflight;
-    if (q >= lim) KAM_HIP(hipEventSynchronize(e->stepEvents[q - lim]));
+    if (q >= lim) KAM_HIP(hipEventSynchronize(e->stepEvents[q - lim].end));
     if (e->dirty) {
         int rc = rebuild(e);
         if (rc != KAM_OK) return rc;
     }
     if (e->active.empty()) return KAM_OK;
+    // an idle GPU: time this step from its own start, not from the last step's end
+    hipEvent_t startEv = nullptr;
+    if (e->stepEvents.empty() || hipEventQuery(e->stepEvents.back().end) == hipSuccess) {
+        startEv = take_event(e);
+        if (!startEv) return set_err(KAM_EHIP, "event creation failed");
+        KAM_HIP(hipEventRecord(startEv, e->stream));
+    }
     int np = e->parity;
     if (e->ed.use_graph && !kernelEvents) {
         hipGraphExec_t& g = e->graph[e->parity];
@@ -355,7 +370,7 @@ int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
     hipEvent_t ev = take_event(e);
     if (!ev) return set_err(KAM_EHIP, "event creation failed");
     KAM_HIP(hipEventRecord(ev, e->stream));
-    e->stepEvents.push_back(ev);
+    e->stepEvents.push_back({startEv, ev});
     e->steps++;
     e->cellSteps += e->curCells;
     // advance every request; retire the finished ones
@@ -430,7 +445,10 @@ int kam_engine_destroy(kam_engine* e) {
         if (r.host) e->pinned.put(r.host, r.hostBytes);
         if (r.copied) (void)hipEventDestroy(r.copied);
     }
-    for (hipEvent_t ev : e->stepEvents) (void)hipEventDestroy(ev);
+    for (const auto& se : e->stepEvents) {
+        if (se.start) (void)hipEventDestroy(se.start);
+        (void)hipEventDestroy(se.end);
+    }
     for (hipEvent_t ev : e->eventPool) (void)hipEventDestroy(ev);
     if (e->startEvent) (void)hipEventDestroy(e->startEvent);
     if (e->statusEv) (void)hipEventDestroy(e->statusEv);

@@ -18,6 +18,10 @@
 #include <cstdint>
 #include <vector>
 
+// The FBM noise below is a hash of exact FP32 / FP64 operation sequences; no
+// multiply-add may be fused.
+#pragma STDC FP_CONTRACT OFF
+
 #include "kamino.h"
 
 namespace {
@@ -127,6 +131,131 @@ extern "C" int kam_init_fields(int n_theta, int n_phi, double radius, int L, uin
                     v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
                 }
                 rho[(size_t)j * n_phi + i] = (float)v;
+            }
+    return KAM_OK;
+}
+
+// Particle layout of KaminoParticles.cu:20-80 with SplitMix64 in place of C
+// rand(): per particle the phi sign, the theta sign, the phi jitter and the
+// theta jitter, in that order.  The reference keeps these in fReal (FP32).
+extern "C" int kam_particles_seed(int n_theta, double density, uint64_t seed, float* coords,
+                                  int64_t* count) {
+    if (n_theta < 2 || !(density >= 0.0) || !count) return KAM_EINVAL;
+    const float linearDensity = std::sqrt((float)density);
+    const float delta = (float)(kPi / n_theta / linearDensity);
+    const float halfDelta = (float)(delta / 2.0);
+    const unsigned nTh = (unsigned)(linearDensity * (float)n_theta);
+    const unsigned nPh = 2 * nTh;
+    *count = (int64_t)nTh * nPh;
+    if (!coords) return KAM_OK;
+    SplitMix rng{seed};
+    for (unsigned i = 0; i < nPh; ++i)
+        for (unsigned j = 0; j < nTh; ++j) {
+            const float signPhi = (float)rng.uniform() >= 0.5f ? 1.f : -1.f;
+            const float signTheta = (float)rng.uniform() >= 0.5f ? 1.f : -1.f;
+            const float randPhi = signPhi * halfDelta * (float)rng.uniform();
+            const float randTheta = signTheta * halfDelta * (float)rng.uniform();
+            float phi = (float)i * delta + randPhi;
+            float theta = (float)j * delta + randTheta;
+            if (phi < 0.f) phi = 0.f;
+            if (theta < 0.f) theta = 0.f;
+            const size_t idx = (size_t)i * nTh + j;
+            coords[2 * idx] = phi;
+            coords[2 * idx + 1] = theta;
+        }
+    return KAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The reference's default initial velocity (KaminoSolver::initialize_velocity,
+// KaminoInitializer.cu:3-88): curl of FBM value noise.  The noise is a hash
+// of FP32 lattice coordinates, so every operation keeps the reference's types:
+// fReal (FP32) variables, double where a double literal or the vec2 class
+// (vectorUtil.cuh: double components) promotes, and FP32 again on assignment.
+// ---------------------------------------------------------------------------
+namespace {
+
+// KaminoSolver::rand (KaminoInitializer.cu:124-133): frac(sin(43758.5453 a.b))
+float fbm_hash(float x, float y) {
+    const double dot = (double)x * 12.9898 + (double)y * 4.1414;
+    const float d = (float)dot;
+    const float val = (float)std::sin((double)d * 43758.5453);
+    return val - std::floor(val);
+}
+
+// kaminoLerpHost (KaminoInitializer.cu:103-106): (1.0 - a) * from + a * to,
+// the first product in double, the second in FP32
+float fbm_lerp(float from, float to, float alpha) {
+    const float at = alpha * to;
+    return (float)((1.0 - (double)alpha) * (double)from + (double)at);
+}
+
+// interpNoise2D (KaminoInitializer.cu:108-122)
+float fbm_noise(float x, float y) {
+    const float intX = std::floor(x), fractX = x - intX;
+    const float intY = std::floor(y), fractY = y - intY;
+    const float v1 = fbm_hash(intX, intY);
+    const float v2 = fbm_hash(intX + 1.f, intY);
+    const float v3 = fbm_hash(intX, intY + 1.f);
+    const float v4 = fbm_hash(intX + 1.f, intY + 1.f);
+    const float i1 = fbm_lerp(v1, v2, fractX);
+    const float i2 = fbm_lerp(v3, v4, fractX);
+    return fbm_lerp(i1, i2, fractY);
+}
+
+// FBM (KaminoInitializer.cu:86-101): four octaves, persistence 0.5,
+// resolutions 0.15 in x and 0.5 in y, scaled by (1 - 0.5) / 2
+float fbm(float x, float y) {
+    const float resX = 0.15f, resY = 0.5f, persistence = 0.5f;
+    float total = 0.0f;
+    for (int i = 0; i < 4; ++i) {
+        const float freq = (float)std::pow(2.0, i);
+        const float amp = (float)std::pow((double)persistence, i);
+        total += amp * fbm_noise(x * freq / resX, y * freq / resY);
+    }
+    const float a = 1.f - persistence;
+    return a * total / 2.0f;
+}
+
+}  // namespace
+
+extern "C" int kam_init_fbm(int n_theta, int n_phi, double radius, float* u_phi, float* u_theta) {
+    if (n_theta < 2 || n_phi < 2 || !(radius > 0.0)) return KAM_EINVAL;
+    const double kPi2 = 6.28318530717958647692;
+    const float gridLen = (float)(kPi2 / n_phi);  // KaminoSolver.cu:14
+    const float R = (float)radius;
+    const float gain = (float)(4096.0 / n_phi);
+    const float rh = R * gridLen;
+    const float g2 = gridLen / 2;
+    auto uphi_at = [&](float ur_x, float ul_x, size_t j) {
+        const float ur_y = (float)(j + 1) * gridLen, lr_y = (float)j * gridLen;
+        const float n_ur = fbm(ur_x, ur_y), n_lr = fbm(ur_x, lr_y);
+        const float n_ul = fbm(ul_x, ur_y), n_ll = fbm(ul_x, lr_y);
+        const float dyl = (n_ur - n_lr) / rh;
+        const float dyr = (n_ul - n_ll) / rh;
+        const float avg = (float)((dyl + dyr) / 2.0);
+        return avg * gain;
+    };
+    if (u_phi)
+        for (size_t j = 0; j < (size_t)n_theta; ++j) {
+            for (size_t i = 1; i < (size_t)n_phi; ++i)
+                u_phi[j * n_phi + i] = uphi_at((float)i * gridLen + g2, (float)i * gridLen - g2, j);
+            // the phi = 0 seam (KaminoInitializer.cu:36-55): the west corners at 2 pi - h/2
+            u_phi[j * n_phi] = uphi_at(g2, (float)(2 * 3.14159265358979323846 - g2), j);
+        }
+    if (u_theta)
+        for (size_t j = 1; j < (size_t)n_theta; ++j)
+            for (size_t i = 0; i < (size_t)n_phi; ++i) {
+                const float ur_x = (float)(i + 1) * gridLen, ul_x = (float)i * gridLen;
+                const float yu = (float)j * gridLen + g2, yd = (float)j * gridLen - g2;
+                const float n_ur = fbm(ur_x, yu), n_lr = fbm(ur_x, yd);
+                const float n_ul = fbm(ul_x, yu);
+                // ll_y = j h + h/2, the same point as ul (KaminoInitializer.cu:69)
+                const float n_ll = fbm(ul_x, yu);
+                const float du = -1 * (n_ur - n_ul) / rh;
+                const float dd = -1 * (n_lr - n_ll) / rh;
+                const float avg = (float)((du + dd) / 2.0);
+                u_theta[(j - 1) * n_phi + i] = avg * gain;
             }
     return KAM_OK;
 }

@@ -211,18 +211,77 @@ __device__ __forceinline__ float geo_root(float G, float vb, float ub) {
 }
 
 // ---------------------------------------------------------------------------
+// LDS-staged sampling.  A tile stages the three input fields over rows
+// [j0 - kER, j0 + R + kER) and columns [c0 - kEC, c0 + W + kEC) into LDS,
+// wrapped at the phi seam once, while staging.  A sample whose bilinear
+// stencil lies inside the staged block, in rows where the global sampler
+// would take its ordinary (no fold, no pole) branch, reads the four values
+// from LDS with two ds_read2 at the fixed row stride SW; any other sample
+// (departure points beyond the halo near the poles, folds, pole rows) goes
+// to the global sampler.  Both read the same values and interpolate with the
+// same weights, so the choice never changes a result.
+// ---------------------------------------------------------------------------
+constexpr int kER = kAdvHaloRows;  // staged rows above and below the tile
+constexpr int kEC = kAdvHaloCols;  // staged columns west and east of the tile
+
+struct Stage {
+    const float* S;      // staged field k (Kind order: u_phi, u_theta, rho) at S + k * plane
+    int plane;           // SR * SW floats
+    int rlo[3], rhi[3];  // per Kind: staged rows lt in [rlo, rhi) take the LDS path
+    int cmax;            // staged columns lp in [0, cmax) take the LDS path
+};
+
+template <int KIND, int SW>
+__device__ __forceinline__ float sample_st(const Stage& st, Rsrc f, const Grid& g, int bi, int lc,
+                                           float fphi, int bj, int lr, float fth) {
+    const float kp = floorf(fphi), kt = floorf(fth);
+    const int lp = lc + (int)kp, lt = lr + (int)kt;
+    const bool inside = ((unsigned)(lt - st.rlo[KIND]) < (unsigned)(st.rhi[KIND] - st.rlo[KIND])) &
+                        ((unsigned)lp < (unsigned)st.cmax);
+    if (inside) {
+        const float* q = st.S + KIND * st.plane + __mul24(lt, SW) + lp;
+        const float ap = fphi - kp, at = fth - kt;
+        return lerpf(lerpf(q[0], q[1], ap), lerpf(q[SW], q[SW + 1], ap), at);
+    }
+    return sample<KIND>(f, g, bi, fphi, bj, fth);
+}
+
+// advect_node with the samples taken from the staged block where possible;
+// (lc, lr) is node (i, j) in staged coordinates.
+template <int NODE, int SRC, int SW>
+__device__ __forceinline__ float advect_node_st(const Stage& st, Rsrc up, Rsrc ut, Rsrc src,
+                                                const Grid& g, int i, int j, int lc, int lr,
+                                                float cofP, float cofT) {
+    constexpr float pP = kNodeP[NODE] + kShiftP[K_VPHI], pT = kNodeT[NODE] + kShiftT[K_VPHI];
+    constexpr float tP = kNodeP[NODE] + kShiftP[K_VTHETA], tT = kNodeT[NODE] + kShiftT[K_VTHETA];
+    constexpr float sP = kNodeP[NODE] + kShiftP[SRC], sT = kNodeT[NODE] + kShiftT[SRC];
+    const int o = __mul24(lr, SW) + lc;
+    const float u0p = (NODE == K_VPHI) ? st.S[o] : sample_st<K_VPHI, SW>(st, up, g, i, lc, pP, j, lr, pT);
+    const float u0t = (NODE == K_VTHETA) ? st.S[st.plane + o]
+                                         : sample_st<K_VTHETA, SW>(st, ut, g, i, lc, tP, j, lr, tT);
+    float dp = u0p * cofP;
+    float dt = u0t * cofT;
+    const float ump = sample_st<K_VPHI, SW>(st, up, g, i, lc, pP - 0.5f * dp, j, lr, pT - 0.5f * dt);
+    const float umt = sample_st<K_VTHETA, SW>(st, ut, g, i, lc, tP - 0.5f * dp, j, lr, tT - 0.5f * dt);
+    dp = 0.5f * (u0p + ump) * cofP;
+    dt = 0.5f * (u0t + umt) * cofT;
+    return sample_st<SRC, SW>(st, src, g, i, lc, sP - dp, j, lr, sT - dt);
+}
+
+// ---------------------------------------------------------------------------
 // k_advect: one tile = rows [j0, j0 + R) x columns [c0, c0 + W) of one
-// simulation.  With the geometric update on, the tile advects into LDS the
-// u_phi values of rows j0..j0+R and columns c0-1..c0+W, and the u_theta
-// values of rows j0-1..j0+R and columns c0-1..c0+W-1 (the update reads one
-// row either side and one column to the west; the restaggered u_phi one more
-// column to the east), plus, in a polar tile, u_theta of the pole-adjacent
-// row at the antipodal columns.  It then solves the cubic per cell and
-// restaggers (assignPhiKernel :517-534, assignThetaKernel :537-549) before
-// anything leaves the CU.  Small tiles keep LDS near 15 KiB, so a CU holds
-// eight workgroups, which the latency of the gathers needs.
+// simulation, W <= SW - 2 kEC.  The tile first stages its input block (see
+// Stage).  With the geometric update on, it then advects into LDS the u_phi
+// values of rows j0..j0+R and columns c0-1..c0+W, and the u_theta values of
+// rows j0-1..j0+R and columns c0-1..c0+W-1 (the update reads one row either
+// side and one column to the west; the restaggered u_phi one more column to
+// the east), plus, in a polar tile, u_theta of the pole-adjacent row at the
+// antipodal columns.  It then solves the cubic per cell and restaggers
+// (assignPhiKernel :517-534, assignThetaKernel :537-549) before anything
+// leaves the CU; the cubic roots reuse the staged block's LDS.
 // Flat item loops split an index into (row, column) with a magic multiply.
 // ---------------------------------------------------------------------------
+template <int SW>
 __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims,
                                                 const Tile* __restrict__ tiles, int ntiles,
                                                 int parity) {
@@ -253,45 +312,77 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
             return i - ((i >= nP) ? nP : 0);
         };
 
+        // ---- stage rows j0-kER .. j0+R+kER-1, columns c0-kEC .. c0+W+kEC-1
+        const int SR = R + 2 * kER, SC = W + 2 * kEC, plane = SR * SW;
+        const int js = j0 - kER;
+        Stage st;
+        st.S = lds;
+        st.plane = plane;
+        st.cmax = SC - 1;
+        // ordinary-branch rows: jt >= 1 and jt < pole row (nT-1; nT-2 for u_theta)
+        st.rlo[K_VPHI] = st.rlo[K_VTHETA] = st.rlo[K_CENTERED] = max(0, 1 - js);
+        st.rhi[K_VPHI] = st.rhi[K_CENTERED] = min(SR - 1, nT - 1 - js);
+        st.rhi[K_VTHETA] = min(SR - 1, nT - 2 - js);
+        for (int e = tid; e < SR * SC; e += nthr) {
+            const int r = e / SC, c = e - r * SC;
+            int gi = c0 - kEC + c;
+            gi += (gi < 0) ? nP : 0;
+            gi -= (gi >= nP) ? nP : 0;
+            // rows outside the field read as 0 (buffer range check) and are never sampled
+            const int o = (js + r) * nP + gi, l = r * SW + c;
+            lds[l] = ldf(up, o);
+            lds[plane + l] = ldf(ut, o);
+            lds[2 * plane + l] = ldf(rh, o);
+        }
+        __syncthreads();
+
         if (!(s.flags & SIM_GEOMETRIC)) {
             for (int idx = tid; idx < R * W; idx += nthr) {
                 const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
                 const int i = c0 + c, j = j0 + r, o = j * nP + i;
-                oup[o] = advect_node<K_VPHI, K_VPHI>(up, ut, up, g, i, j, cofPC[j], cofT);
-                if (j < nT - 1) out[o] = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, g, i, j, cofPT[j], cofT);
-                const float rr = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, g, i, j, cofPC[j], cofT);
+                const int lc = c + kEC, lr = r + kER;
+                oup[o] = advect_node_st<K_VPHI, K_VPHI, SW>(st, up, ut, up, g, i, j, lc, lr, cofPC[j], cofT);
+                if (j < nT - 1)
+                    out[o] = advect_node_st<K_VTHETA, K_VTHETA, SW>(st, up, ut, ut, g, i, j, lc, lr, cofPT[j], cofT);
+                const float rr =
+                    advect_node_st<K_CENTERED, K_CENTERED, SW>(st, up, ut, rh, g, i, j, lc, lr, cofPC[j], cofT);
                 orh[o] = rr;
                 flag_bad(s.status, !isfinite(rr));
             }
+            __syncthreads();
             continue;
         }
 
         const int uLo = j0, uHi = min(j0 + R, nT - 1);
         const int vLo = max(j0 - 1, 0), vHi = min(j0 + R, nT - 2);
-        float* U = lds;                  // (R+1) x W2: u_phi*, columns c0-1 .. c0+W
+        float* U = lds + 3 * plane;      // (R+1) x W2: u_phi*, columns c0-1 .. c0+W
         float* V = U + (R + 1) * W2;     // (R+2) x W1: u_theta*, columns c0-1 .. c0+W-1
-        float* X = V + (R + 2) * W1;     // R x W1: cubic roots, columns c0-1 .. c0+W-1
-        float* A0 = X + R * W1;          // W1: u_theta* of row 0 at the antipodal columns
+        float* A0 = V + (R + 2) * W1;    // W1: u_theta* of row 0 at the antipodal columns
         float* AL = A0 + W1;             // W1: u_theta* of row nT-2 at the antipodal columns
+        float* X = lds;                  // R x W1: cubic roots, over the staged block once it is dead
         const bool north = (j0 == 0), south = (uHi == nT - 1);
         const int half = g.half;
         const int nU = (uHi - uLo + 1) * W2, nV = (vHi - vLo + 1) * W1, nR = R * W;
-        // two independent backtraces per iteration give the gathers more to overlap
+        // two independent backtraces per iteration give the loads more to overlap
         for (int idx = tid; idx < nU; idx += 2 * nthr) {
             const int idx2 = min(idx + nthr, nU - 1);
-            const int r = (int)__umulhi((unsigned)idx, mW2), i = col(idx - r * W2 - 1), j = uLo + r;
-            const int r2 = (int)__umulhi((unsigned)idx2, mW2), i2 = col(idx2 - r2 * W2 - 1), j2 = uLo + r2;
-            const float a = advect_node<K_VPHI, K_VPHI>(up, ut, up, g, i, j, cofPC[j], cofT);
-            const float b = advect_node<K_VPHI, K_VPHI>(up, ut, up, g, i2, j2, cofPC[j2], cofT);
+            const int r = (int)__umulhi((unsigned)idx, mW2), c = idx - r * W2 - 1, j = uLo + r;
+            const int r2 = (int)__umulhi((unsigned)idx2, mW2), c2 = idx2 - r2 * W2 - 1, j2 = uLo + r2;
+            const float a = advect_node_st<K_VPHI, K_VPHI, SW>(st, up, ut, up, g, col(c), j, c + kEC,
+                                                               j - js, cofPC[j], cofT);
+            const float b = advect_node_st<K_VPHI, K_VPHI, SW>(st, up, ut, up, g, col(c2), j2, c2 + kEC,
+                                                               j2 - js, cofPC[j2], cofT);
             U[idx] = a;
             U[idx2] = b;
         }
         for (int idx = tid; idx < nV; idx += 2 * nthr) {
             const int idx2 = min(idx + nthr, nV - 1);
-            const int r = (int)__umulhi((unsigned)idx, mW1), i = col(idx - r * W1 - 1), j = vLo + r;
-            const int r2 = (int)__umulhi((unsigned)idx2, mW1), i2 = col(idx2 - r2 * W1 - 1), j2 = vLo + r2;
-            const float a = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, g, i, j, cofPT[j], cofT);
-            const float b = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, g, i2, j2, cofPT[j2], cofT);
+            const int r = (int)__umulhi((unsigned)idx, mW1), c = idx - r * W1 - 1, j = vLo + r;
+            const int r2 = (int)__umulhi((unsigned)idx2, mW1), c2 = idx2 - r2 * W1 - 1, j2 = vLo + r2;
+            const float a = advect_node_st<K_VTHETA, K_VTHETA, SW>(st, up, ut, ut, g, col(c), j, c + kEC,
+                                                                   j - js, cofPT[j], cofT);
+            const float b = advect_node_st<K_VTHETA, K_VTHETA, SW>(st, up, ut, ut, g, col(c2), j2,
+                                                                   c2 + kEC, j2 - js, cofPT[j2], cofT);
             V[idx] = a;
             V[idx2] = b;
         }
@@ -309,12 +400,14 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         }
         for (int idx = tid; idx < nR; idx += 2 * nthr) {
             const int idx2 = min(idx + nthr, nR - 1);
-            const int r = (int)__umulhi((unsigned)idx, mW), i = c0 + idx - r * W, j = j0 + r;
-            const int r2 = (int)__umulhi((unsigned)idx2, mW), i2 = c0 + idx2 - r2 * W, j2 = j0 + r2;
-            const float a = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, g, i, j, cofPC[j], cofT);
-            const float b = advect_node<K_CENTERED, K_CENTERED>(up, ut, rh, g, i2, j2, cofPC[j2], cofT);
-            orh[j * nP + i] = a;
-            orh[j2 * nP + i2] = b;
+            const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W, j = j0 + r;
+            const int r2 = (int)__umulhi((unsigned)idx2, mW), c2 = idx2 - r2 * W, j2 = j0 + r2;
+            const float a = advect_node_st<K_CENTERED, K_CENTERED, SW>(st, up, ut, rh, g, c0 + c, j, c + kEC,
+                                                                       r + kER, cofPC[j], cofT);
+            const float b = advect_node_st<K_CENTERED, K_CENTERED, SW>(st, up, ut, rh, g, c0 + c2, j2,
+                                                                       c2 + kEC, r2 + kER, cofPC[j2], cofT);
+            orh[j * nP + c0 + c] = a;
+            orh[j2 * nP + c0 + c2] = b;
             flag_bad(s.status, !isfinite(a) || !isfinite(b));
         }
         __syncthreads();
@@ -353,9 +446,65 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
 }
 
 // ---------------------------------------------------------------------------
-// Row FFT: Stockham autosort passes over rows held in LDS, radix 2/3/4/5,
-// every thread doing whole butterflies; a pass reads its inputs into
-// registers, synchronises, then writes its outputs in natural order.
+// Particles (advectionParticles, KaminoCore.cu:321-342): one forward-Euler
+// step of each particle with the pre-step velocities, sampled at the particle
+// by the u_phi and u_theta samplers; no phi motion where R sin(theta) <= 1e-7;
+// then validateCoord.  The fold keeps the reference's arithmetic: FP32
+// positions, the 2 pi and pi constants and the remainder in FP64.  A tile is
+// a run of particles of one simulation; each thread moves one at a time.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void validate_coord(float& phi, float& theta) {
+    constexpr double k2Pi = 6.28318530717958647692, kPiD = 3.14159265358979323846;
+    theta = (float)((double)theta - (double)(int)floorf((float)((double)theta / k2Pi)) * k2Pi);
+    if ((double)theta > kPiD) {
+        theta = (float)(k2Pi - (double)theta);
+        phi = (float)((double)phi + kPiD);
+    }
+    if (theta < 0.f) {
+        theta = -theta;
+        phi = (float)((double)phi + kPiD);
+    }
+    phi = (float)((double)phi - (double)(int)floorf((float)((double)phi / k2Pi)) * k2Pi);
+}
+
+__global__ __launch_bounds__(256) void k_particles(const SimDesc* __restrict__ sims,
+                                                   const Tile* __restrict__ tiles, int ntiles,
+                                                   int parity) {
+    for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
+        const Tile tile = tiles[t];
+        const SimDesc& s = sims[tile.sim];
+        const Grid g{s.nTheta, s.nPhi, s.nPhi >> 1, s.invNP};
+        const int N = g.nT * g.nP;
+        const Rsrc up = make_rsrc(s.state[parity][F_UPHI], N);
+        const Rsrc ut = make_rsrc(s.state[parity][F_UTHETA], N - g.nP);
+        const float2* __restrict__ in = reinterpret_cast<const float2*>(s.part[parity]);
+        float2* __restrict__ outp = reinterpret_cast<float2*>(s.part[parity ^ 1]);
+        const float invH = s.invH, R = s.radius, dt = s.dt;
+        const float cofTheta = dt / R;
+        const int end = tile.row0 + tile.rows;
+        for (int k = tile.row0 + (int)threadIdx.x; k < end; k += (int)blockDim.x) {
+            const float2 p = in[k];
+            // offset-shifted coordinates in cells: u_phi nodes sit at
+            // ((i - 1/2) h, (j + 1/2) h), u_theta nodes at (i h, (j + 1) h)
+            const float x = p.x * invH, y = p.y * invH;
+            const float uP = sample<K_VPHI>(up, g, 0, x + 0.5f, 0, y - 0.5f);
+            const float uT = sample<K_VTHETA>(ut, g, 0, x, 0, y - 1.f);
+            const float latRadius = R * sinf(p.y);
+            float th = p.y + uT * cofTheta;
+            float ph = p.x;
+            if (latRadius > 1e-7f) ph = p.x + uP * (dt / latRadius);
+            validate_coord(ph, th);
+            outp[k] = make_float2(ph, th);
+            flag_bad(s.status, !isfinite(ph) || !isfinite(th));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row FFT, one wavefront per row: Stockham autosort passes (radix 2/3/4/5)
+// ping-pong between the row and the wavefront's own scratch row in LDS.  LDS
+// operations of one wavefront complete in issue order, so consecutive passes
+// are separated by a wavefront-scope fence only, never a workgroup barrier.
 // DIR = -1 for exp(-i ...), +1 for exp(+i ...); unnormalised.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -407,69 +556,87 @@ __device__ __forceinline__ void butterfly(float2* x) {
     }
 }
 
-constexpr int kFftMaxVals = 32;  // complex values per thread per pass: rows * M <= 32 * 256
-
-template <int RADIX, int DIR>
-__device__ void fft_pass(float2* buf, int nrows, int M, int Ls, const float2* __restrict__ twN,
-                         int twScale) {
-    constexpr int MAXB = kFftMaxVals / RADIX;
-    const int tid = threadIdx.x, nthr = blockDim.x;
-    const int nb = M / RADIX;
-    const int total = nrows * nb;
-    float2 v[MAXB][RADIX];
-    int dst[MAXB];
-#pragma unroll
-    for (int c = 0; c < MAXB; ++c) {
-        const int b = tid + c * nthr;
-        dst[c] = -1;
-        if (b < total) {
-            const int row = b / nb, bb = b - row * nb;
-            const int k = bb / Ls, j = bb - k * Ls;
-            const float2* base = buf + (size_t)row * M + j + k * Ls;
-            float2 x[RADIX];
-#pragma unroll
-            for (int q = 0; q < RADIX; ++q) {
-                float2 a = base[q * nb];
-                if (q > 0) {
-                    float2 w = twN[q * j * twScale];
-                    if (DIR > 0) w.y = -w.y;
-                    a = cmul(a, w);
-                }
-                x[q] = a;
-            }
-            butterfly<RADIX, DIR>(x);
-#pragma unroll
-            for (int m = 0; m < RADIX; ++m) v[c][m] = x[m];
-            dst[c] = row * M + j + k * Ls * RADIX;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < MAXB; ++c) {
-        if (dst[c] >= 0) {
-            const int Lsx = Ls;
-#pragma unroll
-            for (int m = 0; m < RADIX; ++m) buf[dst[c] + m * Lsx] = v[c][m];
-        }
-    }
-    __syncthreads();
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lanes per row: a quarter of M rounded up to a power of two, 8..64, so short
+// rows share a wavefront (64 / fft_group lanes groups, one row each).
+__host__ __device__ inline int fft_group(int M) {
+    int g = 8;
+    while (g < 64 && 4 * g < M) g *= 2;
+    return g;
+}
+
+template <int RADIX, int DIR>
+__device__ __forceinline__ void wave_pass(const float2* __restrict__ src, float2* __restrict__ dst,
+                                          int M, int Ls, int lsShift,
+                                          const float2* __restrict__ tw, int twScale, int lane,
+                                          int gs) {
+    const int nb = M / RADIX;
+    for (int b = lane; b < nb; b += gs) {
+        int k, j;
+        if (lsShift >= 0) {
+            k = b >> lsShift;
+            j = b & (Ls - 1);
+        } else {
+            k = b / Ls;
+            j = b - k * Ls;
+        }
+        const float2* in = src + j + k * Ls;
+        float2 x[RADIX];
+#pragma unroll
+        for (int q = 0; q < RADIX; ++q) {
+            float2 v = in[q * nb];
+            if (q > 0) {
+                float2 w = tw[q * j * twScale];
+                if (DIR > 0) w.y = -w.y;
+                v = cmul(v, w);
+            }
+            x[q] = v;
+        }
+        butterfly<RADIX, DIR>(x);
+        float2* out = dst + j + k * Ls * RADIX;
+#pragma unroll
+        for (int m = 0; m < RADIX; ++m) out[m * Ls] = x[m];
+    }
+}
+
+// Transforms the M complex values at `row` (LDS) in place; `scratch` is the
+// calling lane group's own M-entry LDS row, `lane` the lane within the group
+// of gs lanes.  Every lane of the wavefront calls it (rowless groups pass
+// active = false) so that the fences stay wavefront-uniform.
 template <int DIR>
-__device__ void fft_rows(float2* buf, int nrows, const SimDesc& s) {
+__device__ void wave_fft(float2* row, float2* scratch, const SimDesc& s, int lane, int gs, bool active) {
+    const float2* __restrict__ tw = reinterpret_cast<const float2*>(s.twN);
     const int M = s.M;
+    float2* src = row;
+    float2* dst = scratch;
     int Ls = 1;
     for (int p = 0; p < s.nfac; ++p) {
         const int r = s.fac[p];
         const int twScale = s.nPhi / (Ls * r);
-        const float2* tw = reinterpret_cast<const float2*>(s.twN);
-        switch (r) {
-            case 2: fft_pass<2, DIR>(buf, nrows, M, Ls, tw, twScale); break;
-            case 3: fft_pass<3, DIR>(buf, nrows, M, Ls, tw, twScale); break;
-            case 4: fft_pass<4, DIR>(buf, nrows, M, Ls, tw, twScale); break;
-            default: fft_pass<5, DIR>(buf, nrows, M, Ls, tw, twScale); break;
+        const int lsShift = ((Ls & (Ls - 1)) == 0) ? __builtin_ctz(Ls) : -1;
+        if (active) {
+            switch (r) {
+                case 2: wave_pass<2, DIR>(src, dst, M, Ls, lsShift, tw, twScale, lane, gs); break;
+                case 3: wave_pass<3, DIR>(src, dst, M, Ls, lsShift, tw, twScale, lane, gs); break;
+                case 4: wave_pass<4, DIR>(src, dst, M, Ls, lsShift, tw, twScale, lane, gs); break;
+                default: wave_pass<5, DIR>(src, dst, M, Ls, lsShift, tw, twScale, lane, gs); break;
+            }
         }
+        wave_sync();
+        float2* tmp = src;
+        src = dst;
+        dst = tmp;
         Ls *= r;
+    }
+    if (src != row) {
+        if (active)
+            for (int i = lane; i < M; i += gs) row[i] = src[i];
+        wave_sync();
     }
 }
 
@@ -484,39 +651,49 @@ __device__ void fft_rows(float2* buf, int nrows, const SimDesc& s) {
 __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sims,
                                                  const Tile* __restrict__ tiles, int ntiles,
                                                  int parity) {
-    extern __shared__ float2 zbuf[];
+    extern __shared__ float2 zlds[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
         const SimDesc& s = sims[tile.sim];
         const int nT = s.nTheta, nP = s.nPhi, M = s.M;
         const int j0 = tile.row0, R = tile.rows;
+        const int gs = fft_group(M), grp = tid / gs, ngrp = nthr / gs, lane = tid - grp * gs;
         const float* __restrict__ up = s.state[parity][F_UPHI];
         const float* __restrict__ ut = s.state[parity][F_UTHETA];
         const float* __restrict__ divS = s.rowf + RT_DIVS * nT;
         const float* __restrict__ divN = s.rowf + RT_DIVN * nT;
         const float* __restrict__ invHs = s.rowf + RT_INVHS * nT;
-        for (int idx = tid; idx < R * M; idx += nthr) {
-            const int r = idx / M, k = idx - r * M, j = j0 + r;
-            const int i0 = 2 * k, i1 = i0 + 1, i2 = (i1 + 1 == nP) ? 0 : i1 + 1;
+        float2* zbuf = zlds;                                       // R rows of M
+        float2* scratch = zlds + (size_t)(R + grp) * M;            // this lane group's row
+        const int nPass = (R + ngrp - 1) / ngrp;                   // uniform trip count
+        for (int it = 0; it < nPass; ++it) {
+            const int r = it * ngrp + grp;
+            const bool active = r < R;
+            const int j = j0 + (active ? r : 0);
             const float* urow = up + (size_t)j * nP;
-            const float uw0 = urow[i0], ue0 = urow[i1], ue1 = urow[i2];
-            float vn0 = 0.f, vn1 = 0.f, vs0 = 0.f, vs1 = 0.f;
-            if (j > 0) {
-                vn0 = ut[(size_t)(j - 1) * nP + i0];
-                vn1 = ut[(size_t)(j - 1) * nP + i1];
-            }
-            if (j < nT - 1) {
-                vs0 = ut[(size_t)j * nP + i0];
-                vs1 = ut[(size_t)j * nP + i1];
-            }
             const float fs = divS[j], fn = divN[j], fi = invHs[j];
-            const float d0 = (fs * vs0 - fn * vn0) + fi * (ue0 - uw0);
-            const float d1 = (fs * vs1 - fn * vn1) + fi * (ue1 - ue0);
-            zbuf[idx] = make_float2(d0, d1);
+            float2* row = zbuf + (size_t)(active ? r : 0) * M;
+            for (int k = lane; active && k < M; k += gs) {
+                const int i0 = 2 * k, i1 = i0 + 1, i2 = (i1 + 1 == nP) ? 0 : i1 + 1;
+                const float uw0 = urow[i0], ue0 = urow[i1], ue1 = urow[i2];
+                float vn0 = 0.f, vn1 = 0.f, vs0 = 0.f, vs1 = 0.f;
+                if (j > 0) {
+                    vn0 = ut[(size_t)(j - 1) * nP + i0];
+                    vn1 = ut[(size_t)(j - 1) * nP + i1];
+                }
+                if (j < nT - 1) {
+                    vs0 = ut[(size_t)j * nP + i0];
+                    vs1 = ut[(size_t)j * nP + i1];
+                }
+                const float d0 = (fs * vs0 - fn * vn0) + fi * (ue0 - uw0);
+                const float d1 = (fs * vs1 - fn * vn1) + fi * (ue1 - ue0);
+                row[k] = make_float2(d0, d1);
+            }
+            wave_sync();
+            wave_fft<-1>(row, scratch, s, lane, gs, active);
         }
         __syncthreads();
-        fft_rows<-1>(zbuf, R, s);
         float2* __restrict__ spec = reinterpret_cast<float2*>(s.spec);
         const float2* __restrict__ tw = reinterpret_cast<const float2*>(s.twN);
         const float scale = 1.0f / (float)nP;
@@ -555,6 +732,15 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
 // reference and its result is discarded (cacheZeroComponents :672-680 and
 // shiftUKernel :682-703 subtract it); its slot carries the Nyquist mode.
 // ---------------------------------------------------------------------------
+// 1/x to FP64 precision: the hardware estimate refined by two Newton steps
+// (each squares the relative error), shorter than an IEEE division sequence.
+__device__ __forceinline__ double rcp64(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
 template <int L>
 __device__ __forceinline__ double shfl_grp(double v, int srcLaneInGroup, int laneBase) {
     return __shfl(v, laneBase + srcLaneInGroup, 64);
@@ -613,13 +799,13 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
                     c = 0.0;
                 }
                 if (k == 0) {
-                    const double inv = 1.0 / b;
+                    const double inv = rcp64(b);
                     al[k] = a * inv;
                     ga[k] = c * inv;
                     dr[k] = dre * inv;
                     di[k] = dim * inv;
                 } else {
-                    const double inv = 1.0 / (b - a * ga[k - 1]);
+                    const double inv = rcp64(b - a * ga[k - 1]);
                     al[k] = -a * al[k - 1] * inv;
                     ga[k] = c * inv;
                     dr[k] = (dre - a * dr[k - 1]) * inv;
@@ -685,7 +871,7 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
                 Dpr = 0.0;
                 Dpi = 0.0;
             }
-            const double k1 = A / Bm, k2 = C / Bp;
+            const double k1 = A * rcp64(Bm), k2 = C * rcp64(Bp);
             const double nA = -Am * k1, nC = -Cp * k2;
             const double nB = B - Cm * k1 - Ap * k2;
             const double nDr = Dr - Dmr * k1 - Dpr * k2;
@@ -696,7 +882,8 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
             Dr = nDr;
             Di = nDi;
         }
-        const double Xr = Dr / B, Xi = Di / B;
+        const double iB = rcp64(B);
+        const double Xr = Dr * iB, Xi = Di * iB;
         const int sprev = (l > 0) ? l - 1 : l;
         double xmr = shfl_grp<L>(Xr, sprev, base), xmi = shfl_grp<L>(Xi, sprev, base);
         if (l == 0) {
@@ -730,7 +917,7 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
 __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ sims,
                                                    const Tile* __restrict__ tiles, int ntiles,
                                                    int parity) {
-    extern __shared__ float2 zbuf[];
+    extern __shared__ float2 zlds[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
@@ -738,6 +925,9 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
         const int nT = s.nTheta, nP = s.nPhi, M = s.M;
         const int j0 = tile.row0, R = tile.rows;
         const int Rp = min(R + 1, nT - j0);  // rows of p needed
+        const int gs = fft_group(M), grp = tid / gs, ngrp = nthr / gs, lane = tid - grp * gs;
+        float2* zbuf = zlds;                                        // Rp rows of M
+        float2* scratch = zlds + (size_t)(Rp + grp) * M;            // this lane group's row
         const float2* __restrict__ spec = reinterpret_cast<const float2*>(s.spec);
         const float2* __restrict__ tw = reinterpret_cast<const float2*>(s.twN);
         for (int idx = tid; idx < Rp * M; idx += nthr) {
@@ -758,7 +948,12 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
             zbuf[(size_t)r * M + n] = Y;
         }
         __syncthreads();
-        fft_rows<+1>(zbuf, Rp, s);
+        const int nPass = (Rp + ngrp - 1) / ngrp;
+        for (int it = 0; it < nPass; ++it) {
+            const int r = it * ngrp + grp;
+            wave_fft<+1>(zbuf + (size_t)(r < Rp ? r : 0) * M, scratch, s, lane, gs, r < Rp);
+        }
+        __syncthreads();
         const float* p = reinterpret_cast<const float*>(zbuf);  // p[r][i], row stride nP
         float* __restrict__ up = s.state[parity][F_UPHI];
         float* __restrict__ ut = s.state[parity][F_UTHETA];
@@ -793,10 +988,24 @@ static inline int grid_for(int ntiles, int cap) {
 }
 
 hipError_t launch_advect(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
-                         size_t ldsBytes, int gridCap, hipStream_t st) {
+                         int stride, size_t ldsBytes, int gridCap, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_advect, dim3(grid_for(ntiles, gridCap)), dim3(256), ldsBytes, st, sims,
-                       tiles, ntiles, parity);
+    if (stride == adv_stride(64))
+        hipLaunchKernelGGL(k_advect<adv_stride(64)>, dim3(grid_for(ntiles, gridCap)), dim3(256),
+                           ldsBytes, st, sims, tiles, ntiles, parity);
+    else if (stride == adv_stride(128))
+        hipLaunchKernelGGL(k_advect<adv_stride(128)>, dim3(grid_for(ntiles, gridCap)), dim3(256),
+                           ldsBytes, st, sims, tiles, ntiles, parity);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_particles(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
+                            int gridCap, hipStream_t st) {
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_particles, dim3(grid_for(ntiles, gridCap)), dim3(256), 0, st, sims, tiles,
+                       ntiles, parity);
     return hipGetLastError();
 }
 
@@ -839,8 +1048,11 @@ hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, 
 namespace kam {
 
 hipError_t configure_kernels(size_t bytes) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_advect,
+    hipError_t e = hipFuncSetAttribute((const void*)k_advect<adv_stride(64)>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute((const void*)k_advect<adv_stride(128)>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
     e = hipFuncSetAttribute((const void*)k_div_fft, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)bytes);

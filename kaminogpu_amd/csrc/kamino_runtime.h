@@ -88,9 +88,10 @@ private:
     hipEvent_t uploaded_ = nullptr;
     std::vector<Tile> hTiles_;
     int nsims_ = 0;
-    int advOff_ = 0, nAdv_ = 0, fftOff_ = 0, nFft_ = 0;
+    int advOff_ = 0, nAdv_ = 0, fftOff_ = 0, nFft_ = 0, partOff_ = 0, nPart_ = 0;
     std::vector<TriClass> tri_;
     size_t advLds_ = 0, fftLds_ = 0;
+    int advStride_ = 0;  // staged row stride of the advection launch
 };
 
 // Kernel-time categories reported by profiling.

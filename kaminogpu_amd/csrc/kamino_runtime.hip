@@ -50,11 +50,26 @@ static int factor_m(int M, int* fac) {
     return M == 1 ? n : -1;
 }
 
-// LDS of a geometric advection tile of R rows and W columns (k_advect)
-static size_t adv_lds(int R, int W) {
-    return (size_t)((R + 1) * (W + 2) + (R + 2) * (W + 1) + R * (W + 1) + 2 * (W + 1)) * sizeof(float);
+// LDS of an advection tile of R rows and W columns at staged row stride SW
+// (k_advect): the staged input block, plus with the geometric update the
+// advected u_phi and u_theta rows and the antipodal rows (the cubic roots
+// reuse the staged block)
+static size_t adv_lds(int R, int W, int SW, bool geometric) {
+    size_t n = (size_t)3 * (R + 2 * kAdvHaloRows) * SW;
+    if (geometric) n += (size_t)(R + 1) * (W + 2) + (R + 2) * (W + 1) + 2 * (W + 1);
+    return n * sizeof(float);
 }
-static size_t fft_lds(int R, int M) { return (size_t)(R + 1) * M * 2 * sizeof(float); }
+// Lanes per row in the row FFT (a quarter of M, 8..64, power of two); a
+// 256-thread block holds 256 / group lane groups, each with a scratch row.
+static int fft_group_host(int M) {
+    int g = 8;
+    while (g < 64 && 4 * g < M) g *= 2;
+    return g;
+}
+// LDS of an FFT tile: R + 1 rows of M complex plus the lane groups' scratch rows
+static size_t fft_lds(int R, int M) {
+    return (size_t)(R + 1 + 256 / fft_group_host(M)) * M * 2 * sizeof(float);
+}
 
 // ---------------------------------------------------------------------------
 // Tuning table: tile shapes per n_theta, set by the autotuner (or by hand).
@@ -89,16 +104,14 @@ int make_plan(int nT, int nP, Plan& p) {
     p.nfac = factor_m(p.M, p.fac);
     if (p.nfac < 0)
         return set_err(KAM_EINVAL, "n_theta must factor as 2^a 3^b 5^c (got %d)", nT);
-    // advection tile: 8 rows x 128 columns (about 15 KiB of LDS, so eight
+    // advection tile: 8 rows x 128 columns (about 33 KiB of LDS, so four
     // workgroups fit a CU); narrower spheres use their full width
-    p.advCols = nP < 128 ? nP : 128;
+    p.advCols = nP < kAdvMaxCols ? nP : kAdvMaxCols;
     p.advRows = nT < 8 ? nT : 8;
-    // FFT tile: R * M <= 4096 complex, plus one halo row for the inverse
-    int Rf = 4096 / p.M;
-    Rf = Rf < 1 ? 1 : Rf;
+    // FFT tile: R rows (+1 halo row for the inverse, + scratch rows) in ~48 KiB
+    int Rf = (int)(49152 / (8 * (size_t)p.M)) - 1 - 256 / fft_group_host(p.M);
+    Rf = Rf < 1 ? 1 : (Rf > 16 ? 16 : Rf);
     p.fftRows = Rf > nT ? nT : Rf;
-    if ((size_t)(p.fftRows + 1) * p.M > 7680) p.fftRows = 7680 / p.M - 1;
-    if (p.fftRows < 1) return set_err(KAM_EINVAL, "n_phi %d too large for the row FFT", nP);
     // tridiagonal: L lanes per mode, at most K rows per lane
     if (nT <= 64) { p.triL = 8; p.triK = 8; }
     else if (nT <= 128) { p.triL = 16; p.triK = 8; }
@@ -107,27 +120,29 @@ int make_plan(int nT, int nP, Plan& p) {
     else { p.triL = 64; p.triK = 16; }
     // tuned shapes, then manual overrides for tile sweeps
     const TuneEntry te = tune_get(nT);
-    if (te.advCols >= 8 && te.advCols <= 254) p.advCols = te.advCols < nP ? te.advCols : nP;
+    if (te.advCols >= 8 && te.advCols <= kAdvMaxCols) p.advCols = te.advCols < nP ? te.advCols : nP;
     if (te.advRows > 0 || te.fftRows > 0) adjust_plan(p, te.advRows, te.fftRows);
     if (const char* v = std::getenv("KAM_ADV_COLS")) {
         const int w = std::atoi(v);
-        if (w >= 8 && w <= 254) p.advCols = w < nP ? w : nP;
+        if (w >= 8 && w <= kAdvMaxCols) p.advCols = w < nP ? w : nP;
     }
     if (const char* v = std::getenv("KAM_ADV_ROWS")) adjust_plan(p, std::atoi(v), 0);
     if (const char* v = std::getenv("KAM_FFT_ROWS")) adjust_plan(p, 0, std::atoi(v));
-    p.advLds = adv_lds(p.advRows, p.advCols);
+    p.advLds = adv_lds(p.advRows, p.advCols, adv_stride(p.advCols), true);
     p.fftLds = fft_lds(p.fftRows, p.M);
     return KAM_OK;
 }
 
 int adjust_plan(Plan& p, int advRows, int fftRows) {
     if (advRows > 0) {
-        if (advRows > 64) return set_err(KAM_EINVAL, "advection tile too tall");
-        p.advRows = advRows > p.nT ? p.nT : advRows;
-        p.advLds = adv_lds(p.advRows, p.advCols);
+        const int R = advRows > p.nT ? p.nT : advRows;
+        if (adv_lds(R, p.advCols, adv_stride(kAdvMaxCols), true) > 160 * 1024)
+            return set_err(KAM_EINVAL, "advection tile too tall");
+        p.advRows = R;
+        p.advLds = adv_lds(p.advRows, p.advCols, adv_stride(p.advCols), true);
     }
     if (fftRows > 0) {
-        if ((size_t)(fftRows + 1) * p.M > 7680) return set_err(KAM_EINVAL, "FFT tile too tall");
+        if (fft_lds(fftRows, p.M) > 160 * 1024) return set_err(KAM_EINVAL, "FFT tile too tall");
         p.fftRows = fftRows > p.nT ? p.nT : fftRows;
         p.fftLds = fft_lds(p.fftRows, p.M);
     }
@@ -228,6 +243,8 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
     d.magicP = (unsigned)((((uint64_t)1 << 32) + (uint64_t)p.nP - 1) / (uint64_t)p.nP);
     d.invNP = 1.0f / (float)p.nP;
     d.advCols = p.advCols;
+    d.radius = (float)sd.radius;
+    d.dt = (float)sd.dt;
     for (int k = 0; k < 3; ++k) {
         const uint64_t w = (uint64_t)p.advCols + k;
         d.magicW[k] = (unsigned)((((uint64_t)1 << 32) + w - 1) / w);
@@ -298,6 +315,8 @@ int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>
     hTiles_.clear();
     tri_.clear();
     advLds_ = fftLds_ = 0;
+    advStride_ = adv_stride(8);
+    for (const Plan& p : plans) advStride_ = std::max(advStride_, adv_stride(p.advCols));
     const int minTiles = 2 * device_cus();
     std::vector<int> advRows(plans.size()), fftRows(plans.size());
     for (size_t s = 0; s < plans.size(); ++s) {
@@ -315,8 +334,8 @@ int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>
             for (int c = 0; c < p.nP; c += p.advCols)
                 hTiles_.push_back(Tile{(int)s, j, std::min(advRows[s], p.nT - j),
                                        std::min(c, p.nP - p.advCols)});
-        if (sims[s].flags & SIM_GEOMETRIC)
-            advLds_ = std::max(advLds_, adv_lds(advRows[s], p.advCols));
+        advLds_ = std::max(advLds_, adv_lds(advRows[s], p.advCols, advStride_,
+                                            (sims[s].flags & SIM_GEOMETRIC) != 0));
     }
     nAdv_ = (int)hTiles_.size();
     fftOff_ = (int)hTiles_.size();
@@ -324,7 +343,7 @@ int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>
         const Plan& p = plans[s];
         for (int j = 0; j < p.nT; j += fftRows[s])
             hTiles_.push_back(Tile{(int)s, j, std::min(fftRows[s], p.nT - j), 0});
-        fftLds_ = std::max(fftLds_, (size_t)(fftRows[s] + 1) * p.M * 2 * sizeof(float));
+        fftLds_ = std::max(fftLds_, fft_lds(fftRows[s], p.M));
     }
     nFft_ = (int)hTiles_.size() - fftOff_;
     // tridiagonal tiles, grouped by kernel class
@@ -341,6 +360,13 @@ int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>
         const int cnt = (int)hTiles_.size() - off;
         if (cnt > 0) tri_.push_back(TriClass{c[0], c[1], off, cnt});
     }
+    // particle tiles: runs of kPartTile particles
+    constexpr int kPartTile = 2048;
+    partOff_ = (int)hTiles_.size();
+    for (size_t s = 0; s < sims.size(); ++s)
+        for (int k = 0; k < sims[s].nPart; k += kPartTile)
+            hTiles_.push_back(Tile{(int)s, k, std::min(kPartTile, sims[s].nPart - k), 0});
+    nPart_ = (int)hTiles_.size() - partOff_;
     int rc = reserve(sims.size(), hTiles_.size(), st);
     if (rc != KAM_OK) return rc;
     // the staging buffer is free once the previous upload has landed (long
@@ -370,7 +396,8 @@ int DeviceBatch::launch(int phases, int parity, hipStream_t st, int* newParity,
     int p = parity;
     if (phases & KAM_PHASE_ADVECT) {
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_ADVECT], st));
-        KAM_HIP(launch_advect(dSims_, dTiles_ + advOff_, nAdv_, p, advLds_, kGridCap, st));
+        KAM_HIP(launch_advect(dSims_, dTiles_ + advOff_, nAdv_, p, advStride_, advLds_, kGridCap, st));
+        KAM_HIP(launch_particles(dSims_, dTiles_ + partOff_, nPart_, p, kGridCap, st));
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_ADVECT + 1], st));
         p ^= 1;
     }
@@ -442,6 +469,7 @@ struct kam_sim {
     hipStream_t stream = nullptr;
     DeviceBatch batch;
     int parity = 0;
+    float* pmem = nullptr;  // particles: 2 copies of nPart float2
 };
 
 extern "C" {
@@ -522,6 +550,7 @@ int kam_sim_destroy(kam_sim* s) {
         (void)hipStreamDestroy(s->stream);
     }
     if (s->mem) (void)hipFree(s->mem);
+    if (s->pmem) (void)hipFree(s->pmem);
     if (s->tables.rowf) release_tables(s->sd.device, s->sd.n_theta, s->sd.dt, s->sd.radius);
     delete s;
     return KAM_OK;
@@ -560,6 +589,49 @@ int kam_sim_get_fields(kam_sim* s, float* u_phi, float* u_theta, float* rho) {
     KAM_HIP(hipStreamSynchronize(s->stream));
     return KAM_OK;
 }
+
+int kam_sim_set_particles(kam_sim* s, const float* coords, int64_t n) {
+    if (!s || n < 0 || (n > 0 && !coords)) return set_err(KAM_EINVAL, "bad particle arguments");
+    if (n > (int64_t)1 << 30) return set_err(KAM_EINVAL, "too many particles (%lld)", (long long)n);
+    KAM_HIP(hipSetDevice(s->sd.device));
+    KAM_HIP(hipStreamSynchronize(s->stream));
+    if (n != s->desc.nPart) {
+        if (s->pmem) KAM_HIP(hipFree(s->pmem));
+        s->pmem = nullptr;
+        s->desc.part[0] = s->desc.part[1] = nullptr;
+        s->desc.nPart = 0;
+        if (n > 0) {
+            if (hipMalloc(&s->pmem, (size_t)n * 4 * sizeof(float)) != hipSuccess)
+                return set_err(KAM_ENOMEM, "hipMalloc of %lld particles failed", (long long)n);
+            s->desc.part[0] = s->pmem;
+            s->desc.part[1] = s->pmem + (size_t)n * 2;
+            s->desc.nPart = (int)n;
+        }
+    }
+    if (n > 0)
+        KAM_HIP(hipMemcpyAsync(s->desc.part[s->parity], coords, (size_t)n * 2 * sizeof(float),
+                               hipMemcpyHostToDevice, s->stream));
+    std::vector<SimDesc> sims{s->desc};
+    std::vector<Plan> plans{s->plan};
+    int rc = s->batch.build(sims, plans, s->stream);
+    if (rc != KAM_OK) return rc;
+    KAM_HIP(hipStreamSynchronize(s->stream));
+    return KAM_OK;
+}
+
+int kam_sim_get_particles(kam_sim* s, float* coords, int64_t n) {
+    if (!s || (n > 0 && !coords)) return set_err(KAM_EINVAL, "bad particle arguments");
+    if (n != s->desc.nPart)
+        return set_err(KAM_EINVAL, "simulation holds %d particles, not %lld", s->desc.nPart, (long long)n);
+    KAM_HIP(hipSetDevice(s->sd.device));
+    if (n > 0)
+        KAM_HIP(hipMemcpyAsync(coords, s->desc.part[s->parity], (size_t)n * 2 * sizeof(float),
+                               hipMemcpyDeviceToHost, s->stream));
+    KAM_HIP(hipStreamSynchronize(s->stream));
+    return KAM_OK;
+}
+
+int64_t kam_sim_particle_count(const kam_sim* s) { return s ? s->desc.nPart : 0; }
 
 int kam_sim_get_spectrum(kam_sim* s, float* spec) {
     if (!s || !spec) return set_err(KAM_EINVAL, "null argument");

@@ -8,12 +8,20 @@
 //   gridPath particlePath densityImage solidImage colorImage
 // followed, here or on the command line, by optional key=value settings:
 //   seed=<u64> L=<int> density=band|noise geometric=0|1 format=bgeo|raw
-//   device=<int> resume=<raw snapshot>
+//   device=<int> particles=0|1 init=fbm|sh resume=<raw snapshot>
+//   resume_particles=<raw particles>
 // The frame schedule is Kamino::run's (KaminoCore.cu:886-904): steps of dt
 // while T < i*DT, then one remainder step (which, as in the reference's
-// kernels, also advances by dt), then frame i is written to <gridPath><i>.bgeo.
-// Particles, A..E (FBM initialiser) and the image inputs are not used: the
-// initial state is the seeded spherical-harmonic flow of kam_init_fields.
+// kernels, also advances by dt), then frame i is written to <gridPath><i>.bgeo
+// and, with particles (on by default, as the reference builds with
+// WRITE_PARTICLES), the particles to <particlePath><i>.bgeo.  The particle
+// count follows particleDensity truncated to an integer, as
+// initParticlesfromPic receives it (KaminoSolver.cu:279).  The initial
+// velocity is the reference's FBM curl noise (kam_init_fbm, init=fbm, the
+// default) or the seeded spherical-harmonic flow of kam_init_fields (init=sh,
+// with seed and L); the density is the band or the seeded noise of
+// kam_init_fields.  A..E and the image inputs are not used, and particle
+// colours are 0 (the reference's colours when it has no colour image).
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -114,37 +122,43 @@ void put_str(std::ofstream& o, const char* s) {
     o.write(s, std::strlen(s));
 }
 
-// Points carry "v" (vector, 3) and "density" (float, 1) besides the position,
-// as write_data_bgeo (KaminoSolver.cu:284-357); one particle-system primitive
-// lists them all.
-bool write_bgeo(const std::string& file, int n, const float* pos, const float* vel,
-                const float* dens) {
+// A point attribute of a .bgeo file: `size` floats per point; vectors are
+// Houdini type 5, plain floats type 0.
+struct Attr {
+    const char* name;
+    int size;
+    const float* data;
+};
+
+// Houdini geometry V5 (the layout Partio::write produces for particles):
+// points with P (and w = 1) plus the attributes, then one particle-system
+// primitive that lists every point.  write_data_bgeo (KaminoSolver.cu:284-357)
+// writes "v" and "density"; write_particles_bgeo (:359-401) "v" and "color".
+bool write_bgeo(const std::string& file, int n, const float* pos, const std::vector<Attr>& attrs) {
     std::ofstream o(file, std::ios::binary);
     if (!o) return false;
     o.write("Bgeo", 4);
     o.put('V');
     put_be32(o, 5);
-    put_be32(o, (uint32_t)n);  // points
-    put_be32(o, 1);            // primitives
-    put_be32(o, 0);            // point groups
-    put_be32(o, 0);            // primitive groups
-    put_be32(o, 2);            // point attributes (besides P)
-    put_be32(o, 0);            // vertex attributes
-    put_be32(o, 0);            // primitive attributes
-    put_be32(o, 0);            // detail attributes
-    put_str(o, "v");
-    put_be16(o, 3);
-    put_be32(o, 5);  // vector
-    for (int k = 0; k < 3; ++k) put_f(o, 0.f);
-    put_str(o, "density");
-    put_be16(o, 1);
-    put_be32(o, 0);  // float
-    put_f(o, 0.f);
+    put_be32(o, (uint32_t)n);             // points
+    put_be32(o, 1);                       // primitives
+    put_be32(o, 0);                       // point groups
+    put_be32(o, 0);                       // primitive groups
+    put_be32(o, (uint32_t)attrs.size());  // point attributes (besides P)
+    put_be32(o, 0);                       // vertex attributes
+    put_be32(o, 0);                       // primitive attributes
+    put_be32(o, 0);                       // detail attributes
+    for (const Attr& a : attrs) {
+        put_str(o, a.name);
+        put_be16(o, (uint16_t)a.size);
+        put_be32(o, a.size == 3 ? 5 : 0);  // vector or float
+        for (int k = 0; k < a.size; ++k) put_f(o, 0.f);
+    }
     for (int p = 0; p < n; ++p) {
         for (int k = 0; k < 3; ++k) put_f(o, pos[3 * p + k]);
         put_f(o, 1.f);  // homogeneous w
-        for (int k = 0; k < 3; ++k) put_f(o, vel[3 * p + k]);
-        put_f(o, dens[p]);
+        for (const Attr& a : attrs)
+            for (int k = 0; k < a.size; ++k) put_f(o, a.data[(size_t)a.size * p + k]);
     }
     put_be32(o, 0x8000);  // particle-system primitive
     put_be32(o, (uint32_t)n);
@@ -157,6 +171,18 @@ bool write_bgeo(const std::string& file, int n, const float* pos, const float* v
     o.put((char)0x00);
     o.put((char)0xff);
     return (bool)o;
+}
+
+// Particle positions on the sphere (mapPToSphere, KaminoSolver.cu:403-409).
+void particle_points(double R, const std::vector<float>& coords, std::vector<float>& pos) {
+    const size_t n = coords.size() / 2;
+    pos.resize(3 * n);
+    for (size_t k = 0; k < n; ++k) {
+        const float phi = coords[2 * k], theta = coords[2 * k + 1];
+        pos[3 * k] = (float)(R * std::sin(theta) * std::cos(phi));
+        pos[3 * k + 1] = (float)(R * std::cos(theta));
+        pos[3 * k + 2] = (float)(R * std::sin(theta) * std::sin(phi));
+    }
 }
 
 // Cell-centred velocity on the sphere (KaminoSolver.cu:305-353, with
@@ -200,6 +226,28 @@ bool write_raw(const std::string& file, int nT, int nP, int64_t step, double t,
     o.write((const char*)ut.data(), ut.size() * 4);
     o.write((const char*)rho.data(), rho.size() * 4);
     return (bool)o;
+}
+
+// Raw particles: "KAMP", int64 count, then (phi, theta) float32 pairs.
+bool write_raw_particles(const std::string& file, const std::vector<float>& coords) {
+    std::ofstream o(file, std::ios::binary);
+    if (!o) return false;
+    const int64_t n = (int64_t)(coords.size() / 2);
+    o.write("KAMP", 4);
+    o.write((const char*)&n, 8);
+    o.write((const char*)coords.data(), coords.size() * 4);
+    return (bool)o;
+}
+
+bool read_raw_particles(const std::string& file, std::vector<float>& coords) {
+    std::ifstream f(file, std::ios::binary);
+    char magic[4];
+    int64_t n = 0;
+    if (!f.read(magic, 4) || std::memcmp(magic, "KAMP", 4) != 0 || !f.read((char*)&n, 8)) return false;
+    if (n < 0 || n > ((int64_t)1 << 30)) return false;
+    coords.resize((size_t)n * 2);
+    f.read((char*)coords.data(), coords.size() * 4);
+    return (bool)f;
 }
 
 bool read_raw(const std::string& file, int nT, int nP, int64_t& step, double& t, std::vector<float>& up,
@@ -249,6 +297,12 @@ int main(int argc, char** argv) {
     const int geometric = std::atoi(get("geometric", "1").c_str());
     const std::string format = get("format", "bgeo");
     const int device = std::atoi(get("device", "0").c_str());
+    const bool particles = std::atoi(get("particles", "1").c_str()) != 0;
+    const std::string init = get("init", "fbm");
+    if (init != "fbm" && init != "sh") {
+        std::cerr << "kamino: init must be fbm or sh\n";
+        return 1;
+    }
     if (!c.densityImage.empty() || !c.colorImage.empty() || !c.solidImage.empty())
         std::cerr << "kamino: image inputs are not supported; using the seeded initial state\n";
 
@@ -275,7 +329,8 @@ int main(int argc, char** argv) {
             return 2;
         }
     } else if (kam_init_fields(nT, nP, c.radius, L, seed, densityKind, up.data(), ut.data(),
-                               rho.data()) != KAM_OK) {
+                               rho.data()) != KAM_OK ||
+               (init == "fbm" && kam_init_fbm(nT, nP, c.radius, up.data(), ut.data()) != KAM_OK)) {
         std::cerr << "kamino: bad initial-state parameters\n";
         return 2;
     }
@@ -283,14 +338,50 @@ int main(int argc, char** argv) {
         std::cerr << "kamino: " << kam_last_error() << "\n";
         return 2;
     }
-    std::vector<float> pos, vel;
+    std::vector<float> coords;
+    if (particles) {
+        if (c.kv.count("resume_particles")) {
+            if (!read_raw_particles(c.kv["resume_particles"], coords)) {
+                std::cerr << "kamino: cannot resume particles from " << c.kv["resume_particles"] << "\n";
+                return 2;
+            }
+        } else {
+            const double perCell = (double)(size_t)(c.particleDensity > 0 ? c.particleDensity : 0);
+            int64_t n = 0;
+            if (kam_particles_seed(nT, perCell, seed, nullptr, &n) != KAM_OK) {
+                std::cerr << "kamino: bad particle density\n";
+                return 2;
+            }
+            coords.resize((size_t)n * 2);
+            kam_particles_seed(nT, perCell, seed, coords.data(), &n);
+        }
+        if (kam_sim_set_particles(sim, coords.data(), (int64_t)(coords.size() / 2)) != KAM_OK) {
+            std::cerr << "kamino: " << kam_last_error() << "\n";
+            return 2;
+        }
+    }
+    std::vector<float> pos, vel, color;
     auto write_frame = [&](int frame) {
         if (kam_sim_get_fields(sim, up.data(), ut.data(), rho.data()) != KAM_OK) return false;
         const std::string file = c.gridPath + std::to_string(frame) + (format == "raw" ? ".kamr" : ".bgeo");
         std::cout << "Writing to: " << file << std::endl;
-        if (format == "raw") return write_raw(file, nT, nP, step, Tstart, up, ut, rho);
-        frame_points(nT, nP, c.radius, up, ut, pos, vel);
-        return write_bgeo(file, nT * nP, pos.data(), vel.data(), rho.data());
+        if (format == "raw") {
+            if (!write_raw(file, nT, nP, step, Tstart, up, ut, rho)) return false;
+        } else {
+            frame_points(nT, nP, c.radius, up, ut, pos, vel);
+            if (!write_bgeo(file, nT * nP, pos.data(), {{"v", 3, vel.data()}, {"density", 1, rho.data()}}))
+                return false;
+        }
+        if (!particles) return true;
+        if (kam_sim_get_particles(sim, coords.data(), (int64_t)(coords.size() / 2)) != KAM_OK) return false;
+        const std::string pfile = c.particlePath + std::to_string(frame) + (format == "raw" ? ".kamp" : ".bgeo");
+        std::cout << "Writing to: " << pfile << std::endl;
+        if (format == "raw") return write_raw_particles(pfile, coords);
+        const size_t n = coords.size() / 2;
+        particle_points(c.radius, coords, pos);
+        vel.assign(3 * n, 0.f);    // write_particles_bgeo writes v = 0
+        color.assign(3 * n, 0.f);  // no colour image: colorBGR is 0
+        return write_bgeo(pfile, (int)n, pos.data(), {{"v", 3, vel.data()}, {"color", 3, color.data()}});
     };
     if (!write_frame(0)) {
         std::cerr << "kamino: cannot write frame 0 (" << kam_last_error() << ")\n";

@@ -5,8 +5,10 @@ Requests arrive on the wall clock; each waits in a FIFO queue until the
 engine admits it (a free slot and room in the HBM arena), then runs its own
 number of steps inside the engine's shared batched steps, and retires when its
 final fields reach the host.  Reported: cell-steps/s over the run, per-step
-latency percentiles (GPU intervals between step ends) and per-request latency
-percentiles (arrival to results on the host).
+latency percentiles (GPU time of each shared step: from the previous step's
+end, or from its own start when the GPU had gone idle), the fraction of the
+wall time the GPU spent in steps, and per-request latency percentiles (arrival
+to results on the host).
 """
 from __future__ import annotations
 
@@ -69,6 +71,28 @@ class StatePool:
         return self.cache[key]
 
 
+def capacity(requests, engine: Engine, pool: StatePool | None = None, steps=50, dt_scale=0.5):
+    """Saturated throughput of a request mix: every request is admitted at once
+    (as far as the engine's budget allows) and stepped together; returns
+    cell-steps/s of GPU step time.  The requests are cancelled afterwards."""
+    pool = pool or StatePool()
+    ids = []
+    for r in requests:
+        rid = engine.admit(r.n_theta, dt_scale / r.n_theta, *pool.get(r.n_theta, r.seed),
+                           steps=steps + 10)
+        if rid is None:
+            break
+        ids.append(rid)
+    engine.step(5)
+    engine.reset_stats()
+    engine.step(steps)
+    st = engine.stats()
+    for rid in ids:
+        engine.cancel(rid)
+    engine.sync()
+    return st.cell_steps / (st.gpu_ms * 1e-3)
+
+
 def serve(requests, rate, engine: Engine, pool: StatePool | None = None, dt_scale=0.5,
           arrival_seed=1, log=None):
     """Runs the stream; `rate` in requests/s.  dt = dt_scale / n_theta."""
@@ -126,6 +150,7 @@ def serve(requests, rate, engine: Engine, pool: StatePool | None = None, dt_scal
         "cell_steps_per_s": st.cell_steps / wall,
         "step_p50_ms": st.step_p50_ms,
         "step_p99_ms": st.step_p99_ms,
+        "gpu_busy": st.gpu_ms * 1e-3 / wall,
         "request_p50_s": float(np.percentile(lat, 50)),
         "request_p99_s": float(np.percentile(lat, 99)),
         "queue_wait_p99_s": float(np.percentile(wait, 99)),

@@ -118,6 +118,39 @@ double ko_sample(int kind, const double *field, int nTheta, int nPhi, double h,
     }
 }
 
+/* advectionParticles (KaminoCore.cu:321-342): one forward-Euler step of each
+ * particle (phi, theta) with the pre-step velocities sampled at the particle
+ * (sampleVPhi, sampleVTheta), no phi motion where R sin(theta) <= 1e-7, then
+ * validateCoord.  amb[k] = 1 when either sample lay within tol cells of a
+ * sampler branch break (see near_break), exact hits included: a particle
+ * clamped to theta = 0 (KaminoParticles.cu:52-55) samples u_theta exactly at
+ * the flipped idx 0 -> 1 change, and there the reference's own FP32 fold
+ * (h - 2 pi + 2 pi) picks the branch by rounding. */
+static int near_break(int kind, int nTheta, double h, double thetaRaw, double tol);
+static int on_break(int kind, int nTheta, double h, double thetaRaw, double tol);
+
+void ko_advect_particles(int nTheta, int nPhi, double radius, double dt, const double *u_phi,
+                         const double *u_theta, int64_t n, const double *pos, double *out,
+                         uint8_t *amb, double tol) {
+    const double h = REF_PI / nTheta; /* KaminoCore.cu:849 */
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < n; ++k) {
+        const double phi = pos[2 * k], theta = pos[2 * k + 1];
+        const double uPhi = ko_sample(KO_VPHI, u_phi, nTheta, nPhi, h, phi, theta, NULL);
+        const double uTheta = ko_sample(KO_VTHETA, u_theta, nTheta, nPhi, h, phi, theta, NULL);
+        const double latRadius = radius * sin(theta);
+        double th = theta + uTheta * (dt / radius);
+        double ph = phi;
+        if (latRadius > 1e-7) ph = phi + uPhi * (dt / latRadius);
+        ko_validate_coord(&ph, &th);
+        out[2 * k] = ph;
+        out[2 * k + 1] = th;
+        if (amb)
+            amb[k] = (uint8_t)(tol > 0.0 && (on_break(KO_VPHI, nTheta, h, theta, tol) ||
+                                             on_break(KO_VTHETA, nTheta, h, theta, tol)));
+    }
+}
+
 /* Unfolded, offset-shifted theta coordinate in cells, and whether it lies
  * within tol of a point where the sampler's branch selection jumps.  The
  * break points follow from the branch structure above: the north fold at 0,
@@ -132,6 +165,15 @@ static int near_break(int kind, int nTheta, double h, double thetaRaw, double to
         const double d = fabs(t - br[k]);
         if (d > 0.0 && d < tol) return 1;
     }
+    return 0;
+}
+
+static int on_break(int kind, int nTheta, double h, double thetaRaw, double tol) {
+    const double t = (thetaRaw - h * kThetaOff[kind]) / h;
+    const double south = (kind == KO_VTHETA) ? (double)(nTheta - 1) : (double)nTheta;
+    const double br[3] = {-1.0, 0.0, south};
+    for (int k = 0; k < 3; ++k)
+        if (fabs(t - br[k]) < tol) return 1;
     return 0;
 }
 
@@ -881,4 +923,101 @@ void ko_init_fields(int nTheta, int nPhi, double radius, int L, uint64_t seed,
     }
     free(psi);
     free(coef);
+}
+
+/* ---- the reference's default initial velocity ---------------------------
+ * KaminoSolver::initialize_velocity and its FBM helpers
+ * (KaminoInitializer.cu:3-133).  The noise hashes FP32 lattice coordinates,
+ * so this part is restated in the reference's own types, not in FP64: fReal
+ * (float) variables, double where a double literal or the vec2 class
+ * (vectorUtil.cuh:12, double components) promotes, float on assignment. */
+static float ref_rand(float ax, float ay) { /* KaminoInitializer.cu:124-133 */
+    const double vecB0 = 12.9898, vecB1 = 4.1414;
+    const double a0 = (double)ax, a1 = (double)ay;
+    const float dotProd = (float)(a0 * vecB0 + a1 * vecB1);
+    const float val = (float)sin((double)dotProd * 43758.5453);
+    return val - floorf(val);
+}
+
+static float ref_lerp_host(float from, float to, float alpha) { /* :103-106 */
+    const double lhs = (1.0 - (double)alpha) * (double)from;
+    const float rhs = alpha * to;
+    return (float)(lhs + (double)rhs);
+}
+
+static float ref_interp_noise(float x, float y) { /* :108-122 */
+    const float intX = floorf(x);
+    const float fractX = x - intX;
+    const float intY = floorf(y);
+    const float fractY = y - intY;
+    const float v1 = ref_rand(intX, intY);
+    const float v2 = ref_rand(intX + 1, intY);
+    const float v3 = ref_rand(intX, intY + 1);
+    const float v4 = ref_rand(intX + 1, intY + 1);
+    const float i1 = ref_lerp_host(v1, v2, fractX);
+    const float i2 = ref_lerp_host(v3, v4, fractX);
+    return ref_lerp_host(i1, i2, fractY);
+}
+
+float ko_fbm(float x, float y) { /* :86-101 */
+    float total = 0.0f;
+    const float resolutionX = (float)0.15, resolutionY = (float)0.5, persistance = (float)0.5;
+    for (int i = 0; i < 4; i++) {
+        const float freq = (float)pow(2.0, (double)i);
+        const float amp = (float)pow((double)persistance, (double)i);
+        total += amp * ref_interp_noise(x * freq / resolutionX, y * freq / resolutionY);
+    }
+    const float a = 1 - persistance;
+    return a * total / 2.0f;
+}
+
+float ko_fbm_hash(float x, float y) { return ref_rand(x, y); }
+
+void ko_init_fbm(int nTheta, int nPhi, double radius_d, float *u_phi, float *u_theta) {
+    const float gridLen = (float)(REF_2PI / (double)nPhi); /* KaminoSolver.cu:14 */
+    const float radius = (float)radius_d;
+    const float gain = (float)(4096.0 / (double)nPhi);
+    /* u_phi, i >= 1 (:11-34) and the seam i = 0 (:36-55) */
+    for (size_t j = 0; j < (size_t)nTheta; ++j)
+        for (size_t i = 0; i < (size_t)nPhi; ++i) {
+            float ur_x, ul_x;
+            if (i == 0) {
+                ur_x = gridLen / 2;
+                ul_x = (float)(2 * REF_PI - (double)(gridLen / 2));
+            } else {
+                ur_x = (float)i * gridLen + gridLen / 2;
+                ul_x = (float)i * gridLen - gridLen / 2;
+            }
+            const float lr_x = ur_x, ll_x = ul_x;
+            const float ur_y = (float)(j + 1) * gridLen, lr_y = (float)j * gridLen;
+            const float ul_y = (float)(j + 1) * gridLen, ll_y = (float)j * gridLen;
+            const float noise_ur = ko_fbm(ur_x, ur_y);
+            const float noise_lr = ko_fbm(lr_x, lr_y);
+            const float noise_ul = ko_fbm(ul_x, ul_y);
+            const float noise_ll = ko_fbm(ll_x, ll_y);
+            const float noiseDy_l = (noise_ur - noise_lr) / (radius * gridLen);
+            const float noiseDy_r = (noise_ul - noise_ll) / (radius * gridLen);
+            const float avgNoise = (float)((double)(noiseDy_l + noiseDy_r) / 2.0);
+            u_phi[j * (size_t)nPhi + i] = avgNoise * gain;
+        }
+    /* u_theta (:57-84), including ll_y = j h + h/2 (:69) */
+    for (size_t j = 1; j < (size_t)nTheta; ++j)
+        for (size_t i = 0; i < (size_t)nPhi; ++i) {
+            const float ur_x = (float)(i + 1) * gridLen;
+            const float ur_y = (float)j * gridLen + gridLen / 2;
+            const float lr_x = (float)(i + 1) * gridLen;
+            const float lr_y = (float)j * gridLen - gridLen / 2;
+            const float ul_x = (float)i * gridLen;
+            const float ul_y = (float)j * gridLen + gridLen / 2;
+            const float ll_x = (float)i * gridLen;
+            const float ll_y = (float)j * gridLen + gridLen / 2;
+            const float noise_ur = ko_fbm(ur_x, ur_y);
+            const float noise_lr = ko_fbm(lr_x, lr_y);
+            const float noise_ul = ko_fbm(ul_x, ul_y);
+            const float noise_ll = ko_fbm(ll_x, ll_y);
+            const float noiseDy_u = -1 * (noise_ur - noise_ul) / (radius * gridLen);
+            const float noiseDy_d = -1 * (noise_lr - noise_ll) / (radius * gridLen);
+            const float avgNoise = (float)((double)(noiseDy_u + noiseDy_d) / 2.0);
+            u_theta[(j - 1) * (size_t)nPhi + i] = avgNoise * gain;
+        }
 }

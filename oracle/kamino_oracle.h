@@ -92,6 +92,13 @@ void ko_advect(int nTheta, int nPhi, double radius, double dt,
                uint8_t *amb_phi, uint8_t *amb_theta, uint8_t *amb_rho,
                double tol, ko_stats *stats);
 
+/* One particle step (advectionParticles, KaminoCore.cu:321-342): pos and out
+ * hold n (phi, theta) pairs in radians; amb (optional, one byte per particle)
+ * flags samples within tol cells of a sampler branch discontinuity. */
+void ko_advect_particles(int nTheta, int nPhi, double radius, double dt, const double *u_phi,
+                         const double *u_theta, int64_t n, const double *pos, double *out,
+                         uint8_t *amb, double tol);
+
 /* Geometric (metric-term) update, in place (KaminoCore.cu:457-583). */
 void ko_geometric(int nTheta, int nPhi, double radius, double dt,
                   double *u_phi, double *u_theta);
@@ -124,6 +131,14 @@ double ko_run(int nTheta, int nPhi, double radius, double dt, int phases,
               double *rho);
 
 int ko_max_threads(void);
+
+/* The reference's default initial velocity (KaminoInitializer.cu:3-133):
+ * FBM value-noise curl, restated in the reference's FP32/FP64 mix because
+ * the noise hashes FP32 coordinates.  ko_fbm is FBM(x, y), ko_fbm_hash the
+ * lattice hash rand(vec2(x, y)). */
+float ko_fbm(float x, float y);
+float ko_fbm_hash(float x, float y);
+void ko_init_fbm(int nTheta, int nPhi, double radius, float *u_phi, float *u_theta);
 
 #ifdef __cplusplus
 }

@@ -71,6 +71,8 @@ def lib():
                                      _dp, _dp, _dp]
         L.ko_advect.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp, _dp,
                                 _dp, _dp, _dp, _u8p, _u8p, _u8p, C.c_double, C.POINTER(Stats)]
+        L.ko_advect_particles.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp,
+                                          C.c_int64, _dp, _dp, _u8p, C.c_double]
         L.ko_geometric.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp]
         L.ko_geometric_mask.argtypes = [C.c_int, C.c_int, _u8p, _u8p, _u8p, _u8p]
         L.ko_divergence.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
@@ -83,6 +85,12 @@ def lib():
         L.ko_run.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                              _dp, _dp, _dp]
         L.ko_max_threads.restype = C.c_int
+        L.ko_fbm.restype = C.c_float
+        L.ko_fbm.argtypes = [C.c_float, C.c_float]
+        L.ko_fbm_hash.restype = C.c_float
+        L.ko_fbm_hash.argtypes = [C.c_float, C.c_float]
+        _fp32 = C.POINTER(C.c_float)
+        L.ko_init_fbm.argtypes = [C.c_int, C.c_int, C.c_double, _fp32, _fp32]
         _lib = L
     return _lib
 
@@ -172,6 +180,18 @@ def advect(n_theta, n_phi, dt, u_phi, u_theta, rho, radius=1.0, tol=1e-5, stats=
     return op, ot, orr, ap, at, ar, st
 
 
+def advect_particles(n_theta, n_phi, dt, u_phi, u_theta, pos, radius=1.0, tol=1e-5):
+    """One particle step; pos is (n, 2) (phi, theta).  Returns (pos*, amb)."""
+    u_phi = np.ascontiguousarray(u_phi, dtype=np.float64)
+    u_theta = np.ascontiguousarray(u_theta, dtype=np.float64)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 2)
+    out = np.empty_like(pos)
+    amb = np.zeros(len(pos), np.uint8)
+    lib().ko_advect_particles(n_theta, n_phi, radius, dt, _p(u_phi), _p(u_theta), len(pos),
+                              _p(pos), _p(out), _pu8(amb), tol)
+    return out, amb
+
+
 def geometric(n_theta, n_phi, dt, u_phi, u_theta, radius=1.0):
     up = np.array(u_phi, dtype=np.float64, order="C")
     ut = np.array(u_theta, dtype=np.float64, order="C")
@@ -227,6 +247,23 @@ def run(n_theta, n_phi, dt, u_phi, u_theta, rho, steps, threads=0, phases=PHASE_
 
 def max_threads() -> int:
     return lib().ko_max_threads()
+
+
+def fbm(x, y):
+    return float(lib().ko_fbm(float(x), float(y)))
+
+
+def fbm_hash(x, y):
+    return float(lib().ko_fbm_hash(float(x), float(y)))
+
+
+def init_fbm(n_theta, n_phi, radius=1.0):
+    """FP32 (u_phi, u_theta) of the reference's FBM initializer."""
+    up = np.empty((n_theta, n_phi), np.float32)
+    ut = np.empty((n_theta - 1, n_phi), np.float32)
+    fp = C.POINTER(C.c_float)
+    lib().ko_init_fbm(n_theta, n_phi, radius, up.ctypes.data_as(fp), ut.ctypes.data_as(fp))
+    return up, ut
 
 
 def rel_l2(a, b, mask=None):

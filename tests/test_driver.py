@@ -62,18 +62,27 @@ def test_driver_frames_match_library_run(tmp_path):
     nt, dt, DT, frames = 32, 0.005, 0.0123, 3
     write_cfg(tmp_path / "c.txt", [1.0, nt, 200, dt, DT, frames, 0, 1, 1, 1, 1,
                                    str(tmp_path / "frame"), str(tmp_path / "part"), "null", "null",
-                                   "null", "seed=5", "L=6", "density=noise", "format=raw"])
+                                   "null", "seed=5", "L=6", "density=noise", "format=raw", "init=sh"])
     r = run([str(tmp_path / "c.txt")], tmp_path)
     assert r.returncode == 0, r.stderr
     assert "Frame 3 is ready" in r.stdout
     steps = frame_steps(dt, DT, frames)
     st = K.init_fields(nt, 6, 5, K.DENSITY_NOISE)
     # the config's dt is read as a C float, as the reference's main.cu:12 does
+    # particleDensity 200 reaches initParticlesfromPic as an integer (KaminoSolver.cu:279)
+    parts = K.particles_seed(nt, 200, 5)
     with K.Sim(nt, float(np.float32(dt))) as s:
         s.set_fields(*st)
+        s.set_particles(parts)
         for i, n in enumerate(steps, start=1):
             s.step(n)
             ref = s.get_fields()
+            rawp = open(tmp_path / f"part{i}.kamp", "rb").read()
+            assert rawp[:4] == b"KAMP"
+            (cnt,) = struct.unpack("<q", rawp[4:12])
+            assert cnt == len(parts)
+            np.testing.assert_array_equal(np.frombuffer(rawp[12:], np.float32).reshape(-1, 2),
+                                          s.get_particles())
             raw = open(tmp_path / f"frame{i}.kamr", "rb").read()
             assert raw[:4] == b"KAMR"
             a, b, k = struct.unpack("<iiq", raw[4:20])
@@ -86,6 +95,28 @@ def test_driver_frames_match_library_run(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_default_init_is_the_reference_fbm(tmp_path):
+    nt, dt, DT = 32, 0.0005, 0.001
+    write_cfg(tmp_path / "c.txt", [1.0, nt, 4, dt, DT, 1, 0, 1, 1, 1, 1,
+                                   str(tmp_path / "frame"), str(tmp_path / "part"), "null", "null",
+                                   "null", "format=raw", "particles=0"])
+    r = run([str(tmp_path / "c.txt")], tmp_path)
+    assert r.returncode == 0, r.stderr
+    up, ut = K.init_fbm(nt)
+    _, _, rho = K.init_fields(nt, 8, 1, K.DENSITY_BAND)
+    raw0 = np.frombuffer(open(tmp_path / "frame0.kamr", "rb").read()[28:], np.float32)
+    N = nt * 2 * nt
+    np.testing.assert_array_equal(raw0[:N].reshape(nt, 2 * nt), up)
+    np.testing.assert_array_equal(raw0[N:2 * N - 2 * nt].reshape(nt - 1, 2 * nt), ut)
+    with K.Sim(nt, float(np.float32(dt))) as s:
+        s.set_fields(up, ut, rho)
+        s.step(sum(frame_steps(dt, DT, 1)))
+        ref = s.get_fields()
+    raw1 = np.frombuffer(open(tmp_path / "frame1.kamr", "rb").read()[28:], np.float32)
+    np.testing.assert_array_equal(raw1[:N].reshape(nt, 2 * nt), ref[0])
+
+
+@pytest.mark.gpu
 def test_driver_bgeo_layout(tmp_path):
     nt = 16
     write_cfg(tmp_path / "c.txt", [2.0, nt, 200, 0.01, 0.02, 1, 0, 1, 1, 1, 1,
@@ -93,10 +124,15 @@ def test_driver_bgeo_layout(tmp_path):
                                    "null"])
     r = run([str(tmp_path / "c.txt")], tmp_path)
     assert r.returncode == 0, r.stderr
-    b = open(tmp_path / "frame1.bgeo", "rb").read()
+    _check_bgeo(open(tmp_path / "frame1.bgeo", "rb").read(), nt * 2 * nt, [("v", 3), ("density", 1)])
+    npart = len(K.particles_seed(nt, 200, 1))
+    _check_bgeo(open(tmp_path / "part1.bgeo", "rb").read(), npart, [("v", 3), ("color", 3)])
+
+
+def _check_bgeo(b, n, attrs):
     assert b[:5] == b"BgeoV"
     ver, npts, nprim, _, _, npattr = struct.unpack(">6i", b[5:29])
-    assert (ver, npts, nprim, npattr) == (5, nt * 2 * nt, 1, 2)
+    assert (ver, npts, nprim, npattr) == (5, n, 1, len(attrs))
     off = 29 + 12  # vertex / primitive / detail attribute counts
     names = []
     for _ in range(npattr):
@@ -105,8 +141,9 @@ def test_driver_bgeo_layout(tmp_path):
         size, _typ = struct.unpack(">Hi", b[off + 2 + ln:off + 8 + ln])
         names.append((name, size))
         off += 8 + ln + 4 * size
-    assert names == [("v", 3), ("density", 1)]
-    pts = np.frombuffer(b[off:off + npts * 8 * 4], ">f4").reshape(npts, 8)
+    assert names == attrs
+    width = 4 + sum(k for _, k in attrs)
+    pts = np.frombuffer(b[off:off + npts * width * 4], ">f4").reshape(npts, width)
     radius = np.linalg.norm(pts[:, :3], axis=1)
     np.testing.assert_allclose(radius, 2.0, rtol=1e-6)  # points on the sphere
     np.testing.assert_allclose(pts[:, 3], 1.0)
