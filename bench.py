@@ -206,7 +206,7 @@ def main():
         log(f"plain: {plain:.3e} cell-steps/s, {p_total / a.plain_steps:.4f} ms/step")
 
     # ---- tile shapes for this sphere size, timed on the device (kam_autotune)
-    tiles, tuned_ms = tune(K, nt, min(B, 32), a.tune_cache)
+    tiles, tuned_ms = tune(K, nt, B, a.tune_cache)
     log(f"tiles {nt}: {tiles}" + ("" if tuned_ms is None else f", autotuned, kernel ms {np.round(tuned_ms, 4)}"))
 
     # ---- batched form: B spheres through the engine, graph-replayed steps

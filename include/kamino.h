@@ -114,6 +114,15 @@ int kam_sim_get_spectrum(kam_sim *sim, float *spec);
 int kam_sim_benchmark(kam_sim *sim, int nsteps, int per_kernel, double *step_ms,
                       double *kernel_ms, double *total_ms);
 
+/* Solves the tridiagonal systems of the current spectrum in place, exactly as
+ * KAM_PHASE_TRI does (call it after KAM_PHASE_DIVFFT), and reports the
+ * relative Poisson residual ||T_n p_n - d_n|| / ||d_n|| of each mode's FP64
+ * solution before it is rounded to FP32 storage: per_mode[M = n_phi/2]
+ * (optional; slot 0 is the Nyquist mode), their maximum and root mean square.
+ * A diagnostic of the solve (no counterpart in the reference, whose crKernel
+ * solves in FP32, tdm.cu:3-96). */
+int kam_sim_poisson_residual(kam_sim *sim, double *per_mode, double *max_rel, double *rms_rel);
+
 /* Waits for all queued work of the simulation. */
 int kam_sim_sync(kam_sim *sim);
 

@@ -31,7 +31,7 @@ ABI_SYMBOLS = [
     "kam_sim_step", "kam_sim_step_phases", "kam_sim_get_spectrum", "kam_sim_sync", "kam_sim_status",
     "kam_sim_bytes", "kam_sim_benchmark", "kam_init_fields", "kam_engine_completed",
     "kam_sim_set_particles", "kam_sim_get_particles", "kam_sim_particle_count", "kam_particles_seed",
-    "kam_init_fbm",
+    "kam_init_fbm", "kam_sim_poisson_residual",
     "kam_tune_set", "kam_tune_get", "kam_autotune", "kam_tune_load",
     "kam_engine_create", "kam_engine_destroy", "kam_engine_admit", "kam_engine_step",
     "kam_engine_poll", "kam_engine_fetch", "kam_engine_cancel", "kam_engine_sync",
@@ -92,6 +92,7 @@ def lib():
     L.kam_sim_particle_count.argtypes = [C.c_void_p]
     L.kam_particles_seed.argtypes = [C.c_int, C.c_double, C.c_uint64, _fp, C.POINTER(C.c_int64)]
     L.kam_init_fbm.argtypes = [C.c_int, C.c_int, C.c_double, _fp, _fp]
+    L.kam_sim_poisson_residual.argtypes = [C.c_void_p, _dpp, _dpp, _dpp]
     _bind_engine(L)
     _lib = L
     return L
@@ -255,6 +256,15 @@ class Sim:
         c = np.empty((n, 2), np.float32)
         check(lib().kam_sim_get_particles(self._h, _ptr(c) if n else None, n))
         return c
+
+    def poisson_residual(self):
+        """Solves the current spectrum in place (as PHASE_TRI) and returns the
+        per-mode relative residual of the FP64 solution, float64 [M]."""
+        r = np.empty(self.n_phi // 2, np.float64)
+        mx, rms = C.c_double(), C.c_double()
+        check(lib().kam_sim_poisson_residual(self._h, r.ctypes.data_as(C.POINTER(C.c_double)),
+                                             C.byref(mx), C.byref(rms)))
+        return r
 
     def sync(self):
         check(lib().kam_sim_sync(self._h))

@@ -64,6 +64,7 @@ struct SimDesc {
     float* part[2];        // particles, float2 (phi, theta) radians [nPart], per parity
     int nPart;             // particle count (0: none)
     float radius, dt;      // FP32, as the reference's constants (KaminoCore.cu:868-872)
+    double* resid;         // per-mode Poisson residual report (diagnostic launches only)
 };
 
 // One unit of work: rows [row0, row0 + rows) of simulation `sim` (or, for the

@@ -12,6 +12,8 @@
 // weights however far the node sits from phi = 0.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kamino_device.h"
 #include "kamino_launch.h"
 
@@ -268,6 +270,44 @@ __device__ __forceinline__ float advect_node_st(const Stage& st, Rsrc up, Rsrc u
     return sample_st<SRC, SW>(st, src, g, i, lc, sP - dp, j, lr, sT - dt);
 }
 
+// Tile-wide fast path: when the staged velocities bound every RK2
+// displacement within the halo (see k_advect) and the tile is clear of the
+// pole rows, every sample is an ordinary bilinear read of the staged block,
+// with no per-sample check.  Same values, same weights as sample_st.
+template <int KIND, int SW>
+__device__ __forceinline__ float sample_fast(const float* __restrict__ S, int plane, int lc,
+                                             float fphi, int lr, float fth) {
+    const float kp = floorf(fphi), kt = floorf(fth);
+    const float* q = S + KIND * plane + __mul24(lr + (int)kt, SW) + lc + (int)kp;
+    const float ap = fphi - kp, at = fth - kt;
+    return lerpf(lerpf(q[0], q[1], ap), lerpf(q[SW], q[SW + 1], ap), at);
+}
+
+template <int NODE, int SRC, int SW>
+__device__ __forceinline__ float advect_node_fast(const float* __restrict__ S, int plane, int lc,
+                                                  int lr, float cofP, float cofT) {
+    constexpr float pP = kNodeP[NODE] + kShiftP[K_VPHI], pT = kNodeT[NODE] + kShiftT[K_VPHI];
+    constexpr float tP = kNodeP[NODE] + kShiftP[K_VTHETA], tT = kNodeT[NODE] + kShiftT[K_VTHETA];
+    constexpr float sP = kNodeP[NODE] + kShiftP[SRC], sT = kNodeT[NODE] + kShiftT[SRC];
+    const int o = __mul24(lr, SW) + lc;
+    const float u0p = (NODE == K_VPHI) ? S[o] : sample_fast<K_VPHI, SW>(S, plane, lc, pP, lr, pT);
+    const float u0t = (NODE == K_VTHETA) ? S[plane + o] : sample_fast<K_VTHETA, SW>(S, plane, lc, tP, lr, tT);
+    float dp = u0p * cofP;
+    float dt = u0t * cofT;
+    const float ump = sample_fast<K_VPHI, SW>(S, plane, lc, pP - 0.5f * dp, lr, pT - 0.5f * dt);
+    const float umt = sample_fast<K_VTHETA, SW>(S, plane, lc, tP - 0.5f * dp, lr, tT - 0.5f * dt);
+    dp = 0.5f * (u0p + ump) * cofP;
+    dt = 0.5f * (u0t + umt) * cofT;
+    return sample_fast<SRC, SW>(S, plane, lc, sP - dp, lr, sT - dt);
+}
+
+// Displacement budgets of the fast path, in cells.  Node columns sit at
+// staged columns kEC-1 .. W+kEC and sample offsets lie within half a cell,
+// so phi departures below 2.5 cells keep the stencil in [0, W + 2 kEC); node
+// rows sit at staged rows kER-1 .. R+kER, so theta departures below 1.5 cells
+// keep it in [0, R + 2 kER).  The margins absorb FP32 rounding.
+constexpr float kFastDPhi = 2.4f, kFastDTheta = 1.4f;
+
 // ---------------------------------------------------------------------------
 // k_advect: one tile = rows [j0, j0 + R) x columns [c0, c0 + W) of one
 // simulation, W <= SW - 2 kEC.  The tile first stages its input block (see
@@ -323,6 +363,19 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         st.rlo[K_VPHI] = st.rlo[K_VTHETA] = st.rlo[K_CENTERED] = max(0, 1 - js);
         st.rhi[K_VPHI] = st.rhi[K_CENTERED] = min(SR - 1, nT - 1 - js);
         st.rhi[K_VTHETA] = min(SR - 1, nT - 2 - js);
+        // max |u_phi|, |u_theta| of the staged block and max phi coefficient of
+        // the tile's node rows (float bits)
+        __shared__ unsigned umax[3];
+        if (tid < 3) umax[tid] = 0u;
+        __syncthreads();
+        if (tid < R + 2) {  // node rows j0-1 .. j0+R, one per thread
+            const int j = j0 - 1 + tid;
+            if (j >= 0 && j < nT) {
+                const float cm = fmaxf(cofPC[j], cofPT[j]);
+                atomicMax(&umax[2], __float_as_uint(cm != cm ? INFINITY : cm));
+            }
+        }
+        float mp = 0.f, mt = 0.f;
         for (int e = tid; e < SR * SC; e += nthr) {
             const int r = e / SC, c = e - r * SC;
             int gi = c0 - kEC + c;
@@ -330,25 +383,59 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
             gi -= (gi >= nP) ? nP : 0;
             // rows outside the field read as 0 (buffer range check) and are never sampled
             const int o = (js + r) * nP + gi, l = r * SW + c;
-            lds[l] = ldf(up, o);
-            lds[plane + l] = ldf(ut, o);
+            const float a = ldf(up, o), b = ldf(ut, o);
+            lds[l] = a;
+            lds[plane + l] = b;
             lds[2 * plane + l] = ldf(rh, o);
+            // fmaxf drops NaN; a non-finite value must force the checked path
+            mp = (fabsf(a) <= mp) ? mp : fabsf(a);
+            mt = (fabsf(b) <= mt) ? mt : fabsf(b);
         }
+        // non-negative floats (and NaN / Inf, above every finite value) order as unsigned
+        atomicMax(&umax[0], __float_as_uint(mp));
+        atomicMax(&umax[1], __float_as_uint(mt));
         __syncthreads();
+        // fast tile: every sample an ordinary bilinear read of the staged block
+        bool fast;
+        {
+            const float Up = __uint_as_float(umax[0]), Ut = __uint_as_float(umax[1]);
+            const float cmax = __uint_as_float(umax[2]);
+            fast = (j0 >= kER + 1) && (j0 + R <= nT - kER - 1) && (Up * cmax <= kFastDPhi) &&
+                   (Ut * cofT <= kFastDTheta);
+        }
+
+        // one backtrace at node (c, j) of the tile (c from -1, j global): the
+        // branch-free staged sampler on fast tiles, the checked one otherwise
+        auto node = [&](auto fastTag, auto nodeTag, int c, int j, float cofP) -> float {
+            constexpr int NK = decltype(nodeTag)::value;
+            const int lc = c + kEC, lr = j - js;
+            if constexpr (decltype(fastTag)::value) {
+                return advect_node_fast<NK, NK, SW>(lds, plane, lc, lr, cofP, cofT);
+            } else {
+                const Rsrc src = (NK == K_VPHI) ? up : ((NK == K_VTHETA) ? ut : rh);
+                return advect_node_st<NK, NK, SW>(st, up, ut, src, g, col(c), j, lc, lr, cofP, cofT);
+            }
+        };
+        using KPhi = std::integral_constant<int, K_VPHI>;
+        using KTheta = std::integral_constant<int, K_VTHETA>;
+        using KRho = std::integral_constant<int, K_CENTERED>;
 
         if (!(s.flags & SIM_GEOMETRIC)) {
-            for (int idx = tid; idx < R * W; idx += nthr) {
-                const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
-                const int i = c0 + c, j = j0 + r, o = j * nP + i;
-                const int lc = c + kEC, lr = r + kER;
-                oup[o] = advect_node_st<K_VPHI, K_VPHI, SW>(st, up, ut, up, g, i, j, lc, lr, cofPC[j], cofT);
-                if (j < nT - 1)
-                    out[o] = advect_node_st<K_VTHETA, K_VTHETA, SW>(st, up, ut, ut, g, i, j, lc, lr, cofPT[j], cofT);
-                const float rr =
-                    advect_node_st<K_CENTERED, K_CENTERED, SW>(st, up, ut, rh, g, i, j, lc, lr, cofPC[j], cofT);
-                orh[o] = rr;
-                flag_bad(s.status, !isfinite(rr));
-            }
+            auto plain = [&](auto F) {
+                for (int idx = tid; idx < R * W; idx += nthr) {
+                    const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
+                    const int j = j0 + r, o = j * nP + c0 + c;
+                    oup[o] = node(F, KPhi{}, c, j, cofPC[j]);
+                    if (j < nT - 1) out[o] = node(F, KTheta{}, c, j, cofPT[j]);
+                    const float rr = node(F, KRho{}, c, j, cofPC[j]);
+                    orh[o] = rr;
+                    flag_bad(s.status, !isfinite(rr));
+                }
+            };
+            if (fast)
+                plain(std::true_type{});
+            else
+                plain(std::false_type{});
             __syncthreads();
             continue;
         }
@@ -363,29 +450,28 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         const bool north = (j0 == 0), south = (uHi == nT - 1);
         const int half = g.half;
         const int nU = (uHi - uLo + 1) * W2, nV = (vHi - vLo + 1) * W1, nR = R * W;
-        // two independent backtraces per iteration give the loads more to overlap
-        for (int idx = tid; idx < nU; idx += 2 * nthr) {
-            const int idx2 = min(idx + nthr, nU - 1);
-            const int r = (int)__umulhi((unsigned)idx, mW2), c = idx - r * W2 - 1, j = uLo + r;
-            const int r2 = (int)__umulhi((unsigned)idx2, mW2), c2 = idx2 - r2 * W2 - 1, j2 = uLo + r2;
-            const float a = advect_node_st<K_VPHI, K_VPHI, SW>(st, up, ut, up, g, col(c), j, c + kEC,
-                                                               j - js, cofPC[j], cofT);
-            const float b = advect_node_st<K_VPHI, K_VPHI, SW>(st, up, ut, up, g, col(c2), j2, c2 + kEC,
-                                                               j2 - js, cofPC[j2], cofT);
-            U[idx] = a;
-            U[idx2] = b;
-        }
-        for (int idx = tid; idx < nV; idx += 2 * nthr) {
-            const int idx2 = min(idx + nthr, nV - 1);
-            const int r = (int)__umulhi((unsigned)idx, mW1), c = idx - r * W1 - 1, j = vLo + r;
-            const int r2 = (int)__umulhi((unsigned)idx2, mW1), c2 = idx2 - r2 * W1 - 1, j2 = vLo + r2;
-            const float a = advect_node_st<K_VTHETA, K_VTHETA, SW>(st, up, ut, ut, g, col(c), j, c + kEC,
-                                                                   j - js, cofPT[j], cofT);
-            const float b = advect_node_st<K_VTHETA, K_VTHETA, SW>(st, up, ut, ut, g, col(c2), j2,
-                                                                   c2 + kEC, j2 - js, cofPT[j2], cofT);
-            V[idx] = a;
-            V[idx2] = b;
-        }
+        // one backtrace per item: the loops end at wavefront granularity, with
+        // no duplicated tail items (occupancy hides the LDS latency)
+        auto advect_all = [&](auto F) {
+            for (int idx = tid; idx < nU; idx += nthr) {
+                const int r = (int)__umulhi((unsigned)idx, mW2), c = idx - r * W2 - 1, j = uLo + r;
+                U[idx] = node(F, KPhi{}, c, j, cofPC[j]);
+            }
+            for (int idx = tid; idx < nV; idx += nthr) {
+                const int r = (int)__umulhi((unsigned)idx, mW1), c = idx - r * W1 - 1, j = vLo + r;
+                V[idx] = node(F, KTheta{}, c, j, cofPT[j]);
+            }
+            for (int idx = tid; idx < nR; idx += nthr) {
+                const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W, j = j0 + r;
+                const float a = node(F, KRho{}, c, j, cofPC[j]);
+                orh[j * nP + c0 + c] = a;
+                flag_bad(s.status, !isfinite(a));
+            }
+        };
+        if (fast)
+            advect_all(std::true_type{});
+        else
+            advect_all(std::false_type{});
         if (north || south) {
             for (int idx = tid; idx < 2 * W1; idx += nthr) {
                 const bool s0 = idx < W1;
@@ -398,45 +484,38 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
                 (s0 ? A0 : AL)[c] = a;
             }
         }
-        for (int idx = tid; idx < nR; idx += 2 * nthr) {
-            const int idx2 = min(idx + nthr, nR - 1);
-            const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W, j = j0 + r;
-            const int r2 = (int)__umulhi((unsigned)idx2, mW), c2 = idx2 - r2 * W, j2 = j0 + r2;
-            const float a = advect_node_st<K_CENTERED, K_CENTERED, SW>(st, up, ut, rh, g, c0 + c, j, c + kEC,
-                                                                       r + kER, cofPC[j], cofT);
-            const float b = advect_node_st<K_CENTERED, K_CENTERED, SW>(st, up, ut, rh, g, c0 + c2, j2,
-                                                                       c2 + kEC, r2 + kER, cofPC[j2], cofT);
-            orh[j * nP + c0 + c] = a;
-            orh[j2 * nP + c0 + c2] = b;
-            flag_bad(s.status, !isfinite(a) || !isfinite(b));
-        }
         __syncthreads();
 
-        // geometricFillKernel per cell, walking down each column so that v'
-        // of the row above stays in a register for the u_theta restagger
+        // geometricFillKernel, one cell per item: the root x and v' = vbar + G x^2
+        // of rows j0..uHi (the row below the tile only feeds the u_theta
+        // restagger), both into the dead staged block
         const float* __restrict__ Gt = s.rowf + RT_G * nT;
-        for (int c = tid; c < W1; c += nthr) {
-            const int i = col(c - 1);
-            float vprev = 0.f;
-            for (int j = j0; j <= uHi; ++j) {
-                const float* Ur = U + (j - uLo) * W2;
-                const float ub = 0.5f * (Ur[c] + Ur[c + 1]);
-                float vb;
-                if (j == 0)
-                    vb = 0.75f * V[(0 - vLo) * W1 + c] + 0.25f * A0[c];
-                else if (j == nT - 1)
-                    vb = 0.75f * V[(nT - 2 - vLo) * W1 + c] + 0.25f * AL[c];
-                else
-                    vb = 0.5f * (V[(j - 1 - vLo) * W1 + c] + V[(j - vLo) * W1 + c]);
-                const float G = Gt[j];
-                const float x = geo_root(G, vb, ub);
-                const float vp = fmaf(G * x, x, vb);
-                if (j < j0 + R) X[(j - j0) * W1 + c] = x;
-                if (j > j0 && c > 0) out[(j - 1) * nP + i] = 0.5f * (vprev + vp);
-                vprev = vp;
-            }
+        const int nG = (uHi - j0 + 1) * W1;
+        float* VP = X + (R + 1) * W1;  // (R+1) x W1: v'
+        for (int idx = tid; idx < nG; idx += nthr) {
+            const int r = (int)__umulhi((unsigned)idx, mW1), c = idx - r * W1, j = j0 + r;
+            const float* Ur = U + (j - uLo) * W2;
+            const float ub = 0.5f * (Ur[c] + Ur[c + 1]);
+            float vb;
+            if (j == 0)
+                vb = 0.75f * V[(0 - vLo) * W1 + c] + 0.25f * A0[c];
+            else if (j == nT - 1)
+                vb = 0.75f * V[(nT - 2 - vLo) * W1 + c] + 0.25f * AL[c];
+            else
+                vb = 0.5f * (V[(j - 1 - vLo) * W1 + c] + V[(j - vLo) * W1 + c]);
+            const float G = Gt[j];
+            const float x = geo_root(G, vb, ub);
+            X[idx] = x;
+            VP[idx] = fmaf(G * x, x, vb);
         }
         __syncthreads();
+        // assignThetaKernel (rows j0..uHi-1, columns c0..c0+W-1) and
+        // assignPhiKernel (rows j0..j0+R-1)
+        const int nT2 = (uHi - j0) * W;
+        for (int idx = tid; idx < nT2; idx += nthr) {
+            const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
+            out[(j0 + r) * nP + c0 + c] = 0.5f * (VP[r * W1 + c + 1] + VP[(r + 1) * W1 + c + 1]);
+        }
         for (int idx = tid; idx < nR; idx += nthr) {
             const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
             oup[(j0 + r) * nP + c0 + c] = 0.5f * (X[r * W1 + c] + X[r * W1 + c + 1]);
@@ -697,8 +776,9 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
         float2* __restrict__ spec = reinterpret_cast<float2*>(s.spec);
         const float2* __restrict__ tw = reinterpret_cast<const float2*>(s.twN);
         const float scale = 1.0f / (float)nP;
+        const unsigned magicR = (unsigned)((((uint64_t)1 << 32) + R - 1) / R);
         for (int idx = tid; idx < R * M; idx += nthr) {
-            const int n = idx / R, r = idx - n * R, j = j0 + r;
+            const int n = (R == 1) ? idx : (int)__umulhi((unsigned)idx, magicR), r = idx - n * R, j = j0 + r;
             const float2 Zn = zbuf[(size_t)r * M + n];
             float2 out;
             if (n == 0) {
@@ -735,10 +815,8 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
 // 1/x to FP64 precision: the hardware estimate refined by two Newton steps
 // (each squares the relative error), shorter than an IEEE division sequence.
 __device__ __forceinline__ double rcp64(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
-    r = fma(r, fma(-x, r, 1.0), r);
-    return r;
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);  // one Newton step doubles the hardware estimate's bits
 }
 
 template <int L>
@@ -746,10 +824,18 @@ __device__ __forceinline__ double shfl_grp(double v, int srcLaneInGroup, int lan
     return __shfl(v, laneBase + srcLaneInGroup, 64);
 }
 
-template <int L, int KMAX>
+// RESID additionally writes, per mode, ||T x - d|| / ||d|| of the FP64
+// solution x before it is rounded to FP32 (to SimDesc::resid).
+template <int L, int KMAX, bool RESID>
 __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sims,
                                                  const Tile* __restrict__ tiles, int ntiles) {
     constexpr int G = 256 / L;  // modes per block
+    // The block's G mode columns and the FP64 row terms pass through LDS:
+    // global reads and writes stay contiguous, and lane l finds its row
+    // l K + k at [k][l] (a padded row per k keeps the staging writes spread
+    // over the banks).
+    __shared__ float2 zs[G][KMAX][L + 1];
+    __shared__ double cs[KMAX][L + 1], ss[KMAX][L + 1];
     const int tid = threadIdx.x;
     const int l = tid % L;               // lane within the mode group
     const int g = tid / L;               // mode within the block
@@ -762,13 +848,28 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
         const bool active = g < tile.rows;
         const int n = tile.row0 + (active ? g : 0);
         const int K = (nT + L - 1) / L;
+        // i / K and e / nT as multiply-high (exact for dividends below 2^16;
+        // K = 1, whose multiplier 2^32 does not fit, is taken apart)
+        const unsigned magicK = (unsigned)((((uint64_t)1 << 32) + K - 1) / K);
+        const unsigned magicT = (unsigned)((((uint64_t)1 << 32) + nT - 1) / nT);
         const int r0 = l * K;
         int cnt = nT - r0;
         cnt = cnt < 0 ? 0 : (cnt > K ? K : cnt);
         if (!active) cnt = 0;
-        float2* __restrict__ col = reinterpret_cast<float2*>(s.spec) + (size_t)n * nT;
+        float2* __restrict__ blk = reinterpret_cast<float2*>(s.spec) + (size_t)tile.row0 * nT;
         const double* __restrict__ cot2h = s.rowd + RD_COT2H * nT;
         const double* __restrict__ isin2 = s.rowd + RD_INVSIN2 * nT;
+        for (int i = tid; i < nT; i += 256) {
+            const int li = (K == 1) ? i : (int)__umulhi((unsigned)i, magicK), ki = i - li * K;
+            cs[ki][li] = cot2h[i];
+            ss[ki][li] = isin2[i];
+        }
+        for (int e = tid; e < tile.rows * nT; e += 256) {
+            const int ge = (int)__umulhi((unsigned)e, magicT), i = e - ge * nT;
+            const int li = (K == 1) ? i : (int)__umulhi((unsigned)i, magicK), ki = i - li * K;
+            zs[ge][ki][li] = blk[e];
+        }
+        __syncthreads();
         const double h2 = s.invH2;
         const double nn = (n == 0) ? (double)s.M * (double)s.M : (double)n * (double)n;
 
@@ -785,11 +886,12 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
         for (int k = 0; k < KMAX; ++k) {
             if (k < cnt) {
                 const int i = r0 + k;
-                const float2 d = col[i];
+                const float2 d = zs[g][k][l];
                 const double dre = (n == 0) ? 0.0 : (double)d.x;
                 const double dim = (double)d.y;
-                double a = h2 - cot2h[i], c = h2 + cot2h[i];
-                double b = -2.0 * h2 - nn * isin2[i];
+                const double ct = cs[k][l];
+                double a = h2 - ct, c = h2 + ct;
+                double b = -2.0 * h2 - nn * ss[k][l];
                 if (i == 0) {
                     b += a;
                     a = 0.0;
@@ -892,6 +994,7 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
         }
         // back substitution: x_i = delta_i - alpha_i x_{s-1} - gamma_i x_{i+1}
         double xr = Xr, xi = Xi;
+        double xsr[RESID ? KMAX : 1], xsi[RESID ? KMAX : 1];
 #pragma unroll
         for (int k = KMAX - 1; k >= 0; --k) {
             if (k < cnt) {
@@ -899,8 +1002,60 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
                     xr = dr[k] - al[k] * xmr - ga[k] * xr;
                     xi = di[k] - al[k] * xmi - ga[k] * xi;
                 }
-                col[r0 + k] = make_float2((n == 0) ? 0.f : (float)xr, (float)xi);
+                if constexpr (RESID) {
+                    xsr[k] = (n == 0) ? 0.0 : xr;
+                    xsi[k] = xi;
+                } else {
+                    zs[g][k][l] = make_float2((n == 0) ? 0.f : (float)xr, (float)xi);
+                }
             }
+        }
+        if constexpr (RESID) {
+            // neighbours across lane boundaries: the next lane's first row
+            const int snext = (l + 1 < L) ? l + 1 : l;
+            const double x0r = shfl_grp<L>(xsr[0], snext, base), x0i = shfl_grp<L>(xsi[0], snext, base);
+            double rr = 0.0, dd = 0.0;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+                if (k < cnt) {
+                    const int i = r0 + k;
+                    const float2 d = zs[g][k][l];
+                    const double dre = (n == 0) ? 0.0 : (double)d.x, dim = (double)d.y;
+                    const double ct = cs[k][l];
+                    double a = h2 - ct, c = h2 + ct;
+                    double b = -2.0 * h2 - nn * ss[k][l];
+                    if (i == 0) {
+                        b += a;
+                        a = 0.0;
+                    }
+                    if (i == nT - 1) {
+                        b += c;
+                        c = 0.0;
+                    }
+                    const double pr = (k == 0) ? xmr : xsr[k > 0 ? k - 1 : 0];
+                    const double pi = (k == 0) ? xmi : xsi[k > 0 ? k - 1 : 0];
+                    const double qr = (k == cnt - 1) ? x0r : xsr[k + 1 < KMAX ? k + 1 : k];
+                    const double qi = (k == cnt - 1) ? x0i : xsi[k + 1 < KMAX ? k + 1 : k];
+                    const double er = dre - (a * pr + b * xsr[k] + c * qr);
+                    const double ei = dim - (a * pi + b * xsi[k] + c * qi);
+                    rr += er * er + ei * ei;
+                    dd += dre * dre + dim * dim;
+                }
+            }
+            for (int m = 1; m < L; m <<= 1) {
+                rr += __shfl_xor(rr, m, 64);
+                dd += __shfl_xor(dd, m, 64);
+            }
+            if (active && l == 0) s.resid[n] = dd > 0.0 ? sqrt(rr / dd) : 0.0;
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k)
+                if (k < cnt) zs[g][k][l] = make_float2((float)xsr[k], (float)xsi[k]);
+        }
+        __syncthreads();
+        for (int e = tid; e < tile.rows * nT; e += 256) {
+            const int ge = (int)__umulhi((unsigned)e, magicT), i = e - ge * nT;
+            const int li = (K == 1) ? i : (int)__umulhi((unsigned)i, magicK), ki = i - li * K;
+            blk[e] = zs[ge][ki][li];
         }
     }
 }
@@ -930,8 +1085,9 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
         float2* scratch = zlds + (size_t)(Rp + grp) * M;            // this lane group's row
         const float2* __restrict__ spec = reinterpret_cast<const float2*>(s.spec);
         const float2* __restrict__ tw = reinterpret_cast<const float2*>(s.twN);
+        const unsigned magicR = (unsigned)((((uint64_t)1 << 32) + Rp - 1) / Rp);
         for (int idx = tid; idx < Rp * M; idx += nthr) {
-            const int n = idx / Rp, r = idx - n * Rp, j = j0 + r;
+            const int n = (Rp == 1) ? idx : (int)__umulhi((unsigned)idx, magicR), r = idx - n * Rp, j = j0 + r;
             const float2 Pn = spec[(size_t)n * nT + j];
             float2 Y;
             if (n == 0) {
@@ -960,7 +1116,7 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
         const float* __restrict__ invHs = s.rowf + RT_INVHS * nT;
         const float invH = s.invH;
         for (int idx = tid; idx < R * nP; idx += nthr) {
-            const int r = idx / nP, i = idx - r * nP, j = j0 + r;
+            const int r = (int)__umulhi((unsigned)idx, s.magicP), i = idx - r * nP, j = j0 + r;
             const int il = (i == 0) ? nP - 1 : i - 1;
             const float pc = p[(size_t)r * nP + i];
             const size_t o = (size_t)j * nP + i;
@@ -1018,13 +1174,16 @@ hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, in
 }
 
 hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles,
-                          int ntiles, int gridCap, hipStream_t st) {
+                          int ntiles, int gridCap, hipStream_t st, bool resid) {
     if (ntiles <= 0) return hipSuccess;
     const dim3 grid(grid_for(ntiles, gridCap)), block(256);
-#define KAM_TRI(LL, KK)                                                                 \
-    if (lanesPerMode == LL && kmax == KK) {                                             \
-        hipLaunchKernelGGL((k_tridiag<LL, KK>), grid, block, 0, st, sims, tiles, ntiles); \
-        return hipGetLastError();                                                       \
+#define KAM_TRI(LL, KK)                                                                        \
+    if (lanesPerMode == LL && kmax == KK) {                                                    \
+        if (resid)                                                                             \
+            hipLaunchKernelGGL((k_tridiag<LL, KK, true>), grid, block, 0, st, sims, tiles, ntiles); \
+        else                                                                                   \
+            hipLaunchKernelGGL((k_tridiag<LL, KK, false>), grid, block, 0, st, sims, tiles, ntiles); \
+        return hipGetLastError();                                                              \
     }
     KAM_TRI(8, 8)
     KAM_TRI(16, 8)
@@ -1047,18 +1206,23 @@ hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, 
 
 namespace kam {
 
+// Raises each kernel's dynamic-LDS limit to `bytes` less its static LDS.
+static hipError_t allow_lds(const void* fn, size_t bytes) {
+    hipFuncAttributes a;
+    hipError_t e = hipFuncGetAttributes(&a, fn);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)(bytes - a.sharedSizeBytes));
+}
+
 hipError_t configure_kernels(size_t bytes) {
-    hipError_t e = hipFuncSetAttribute((const void*)k_advect<adv_stride(64)>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)k_advect<adv_stride(128)>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    if (e != hipSuccess) return e;
-    e = hipFuncSetAttribute((const void*)k_div_fft, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)bytes);
-    if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void*)k_ifft_grad,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    const void* fns[] = {(const void*)k_advect<adv_stride(64)>, (const void*)k_advect<adv_stride(128)>,
+                         (const void*)k_div_fft, (const void*)k_ifft_grad};
+    for (const void* fn : fns) {
+        const hipError_t e = allow_lds(fn, bytes);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace kam

@@ -20,7 +20,7 @@ hipError_t launch_particles(const SimDesc* sims, const Tile* tiles, int ntiles, 
 hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
                           size_t ldsBytes, int gridCap, hipStream_t st);
 hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles,
-                          int ntiles, int gridCap, hipStream_t st);
+                          int ntiles, int gridCap, hipStream_t st, bool resid = false);
 hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
                             size_t ldsBytes, int gridCap, hipStream_t st);
 

@@ -74,6 +74,8 @@ public:
     // returns the parity after the step (flipped iff the advection ran).
     int launch(int phases, int parity, hipStream_t st, int* newParity,
                hipEvent_t* kernelEvents = nullptr);
+    // Queues the tridiagonal phase alone, with the per-mode residual report.
+    int launch_tri_residual(hipStream_t st);
     int nsims() const { return nsims_; }
     size_t capacityBytes() const { return capSims_ * sizeof(SimDesc) + capTiles_ * sizeof(Tile); }
 

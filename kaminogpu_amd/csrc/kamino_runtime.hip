@@ -136,7 +136,7 @@ int make_plan(int nT, int nP, Plan& p) {
 int adjust_plan(Plan& p, int advRows, int fftRows) {
     if (advRows > 0) {
         const int R = advRows > p.nT ? p.nT : advRows;
-        if (adv_lds(R, p.advCols, adv_stride(kAdvMaxCols), true) > 160 * 1024)
+        if (adv_lds(R, p.advCols, adv_stride(kAdvMaxCols), true) > 160 * 1024 - 256)
             return set_err(KAM_EINVAL, "advection tile too tall");
         p.advRows = R;
         p.advLds = adv_lds(p.advRows, p.advCols, adv_stride(p.advCols), true);
@@ -387,6 +387,12 @@ int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>
 
 static const int kGridCap = 1 << 20;
 
+int DeviceBatch::launch_tri_residual(hipStream_t st) {
+    for (const TriClass& c : tri_)
+        KAM_HIP(launch_tridiag(c.L, c.K, dSims_, dTiles_ + c.off, c.count, kGridCap, st, true));
+    return KAM_OK;
+}
+
 int DeviceBatch::launch(int phases, int parity, hipStream_t st, int* newParity,
                         hipEvent_t* ev) {
     if (nsims_ == 0) {
@@ -470,6 +476,7 @@ struct kam_sim {
     DeviceBatch batch;
     int parity = 0;
     float* pmem = nullptr;  // particles: 2 copies of nPart float2
+    double* resid = nullptr;  // per-mode residual report, on first use
 };
 
 extern "C" {
@@ -551,6 +558,7 @@ int kam_sim_destroy(kam_sim* s) {
     }
     if (s->mem) (void)hipFree(s->mem);
     if (s->pmem) (void)hipFree(s->pmem);
+    if (s->resid) (void)hipFree(s->resid);
     if (s->tables.rowf) release_tables(s->sd.device, s->sd.n_theta, s->sd.dt, s->sd.radius);
     delete s;
     return KAM_OK;
@@ -632,6 +640,36 @@ int kam_sim_get_particles(kam_sim* s, float* coords, int64_t n) {
 }
 
 int64_t kam_sim_particle_count(const kam_sim* s) { return s ? s->desc.nPart : 0; }
+
+int kam_sim_poisson_residual(kam_sim* s, double* per_mode, double* max_rel, double* rms_rel) {
+    if (!s) return set_err(KAM_EINVAL, "null simulation");
+    KAM_HIP(hipSetDevice(s->sd.device));
+    const int M = s->plan.M;
+    if (!s->resid) {
+        KAM_HIP(hipStreamSynchronize(s->stream));
+        if (hipMalloc(&s->resid, (size_t)M * sizeof(double)) != hipSuccess)
+            return set_err(KAM_ENOMEM, "residual buffer allocation failed");
+        s->desc.resid = s->resid;
+        std::vector<SimDesc> sims{s->desc};
+        std::vector<Plan> plans{s->plan};
+        int rc = s->batch.build(sims, plans, s->stream);
+        if (rc != KAM_OK) return rc;
+    }
+    int rc = s->batch.launch_tri_residual(s->stream);
+    if (rc != KAM_OK) return rc;
+    std::vector<double> h((size_t)M);
+    KAM_HIP(hipMemcpyAsync(h.data(), s->resid, (size_t)M * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    KAM_HIP(hipStreamSynchronize(s->stream));
+    double mx = 0.0, sq = 0.0;
+    for (double v : h) {
+        mx = v > mx ? v : mx;
+        sq += v * v;
+    }
+    if (per_mode) std::memcpy(per_mode, h.data(), (size_t)M * sizeof(double));
+    if (max_rel) *max_rel = mx;
+    if (rms_rel) *rms_rel = std::sqrt(sq / M);
+    return KAM_OK;
+}
 
 int kam_sim_get_spectrum(kam_sim* s, float* spec) {
     if (!s || !spec) return set_err(KAM_EINVAL, "null argument");

@@ -2,16 +2,21 @@
 import numpy as np
 
 
-def test_magic_division_exact_for_every_supported_width():
-    # k_advect splits a flat item index into (row, column) as
-    # row = umulhi(idx, ceil(2^32 / nPhi)); items stay below 2^16.
+def test_magic_division_exact_for_every_divisor_used():
+    # Flat item loops split an index as q = umulhi(i, ceil(2^32 / d)): tile
+    # widths W, W+1, W+2 (k_advect), K = rows per lane and nTheta (k_tridiag
+    # staging); every divisor 2..2048 (k_tridiag handles K = 1 apart: its
+    # multiplier 2^32 does not fit 32 bits), every dividend below 2^16.
     idx = np.arange(1 << 16, dtype=np.uint64)
-    for n_phi in range(8, 2 * 1024 + 1, 2):
-        m = ((1 << 32) + n_phi - 1) // n_phi
+    for d in range(2, 2049):
+        m = ((1 << 32) + d - 1) // d
         q = (idx * np.uint64(m)) >> np.uint64(32)
-        assert np.array_equal(q, idx // np.uint64(n_phi)), n_phi
+        assert np.array_equal(q, idx // np.uint64(d)), d
 
 
-def test_advection_items_fit_the_magic_range():
-    # the largest flat loop: (R + 2) rows of nPhi with R <= 8, nPhi <= 2048
-    assert (8 + 2) * 2048 < (1 << 16)
+def test_flat_loops_fit_the_magic_range():
+    # k_advect: staged block (R + 6)(W + 8) and u_phi items (R + 1)(W + 2) with
+    # R <= 64, W <= 128; k_tridiag staging: (256 / L) modes x nTheta <= 4096
+    assert (64 + 6) * (128 + 8) < (1 << 16)
+    assert (64 + 1) * (128 + 2) < (1 << 16)
+    assert max((256 // L) * nt for L, nt in [(8, 64), (16, 128), (32, 256), (64, 1024)]) < (1 << 16)
