@@ -55,16 +55,20 @@ struct Grid {
     float invNP;
 };
 
+// A coordinate in cells as its cell index and the offset within the cell
+// (v_cvt_flr_i32_f32 and v_fract_f32).  Every sampling path splits its
+// coordinates this way, so they all interpolate with identical weights.
+__device__ __forceinline__ int cell_of(float x) { return (int)floorf(x); }
+__device__ __forceinline__ float frac_of(float x) { return __builtin_amdgcn_fractf(x); }
+
 template <int KIND>
 __device__ __forceinline__ float sample(Rsrc f, const Grid& g, int bi, float fphi, int bj, float fth) {
     const int rows = (KIND == K_VTHETA) ? g.nT - 1 : g.nT;
     const int poleRow = (KIND == K_VTHETA) ? g.nT - 2 : g.nT - 1;
-    const float kp = floorf(fphi);
-    int ip = bi + (int)kp;
-    const float ap = fphi - kp;
-    const float kt = floorf(fth);
-    int jt = bj + (int)kt;
-    float at = fth - kt;
+    int ip = bi + cell_of(fphi);
+    const float ap = frac_of(fphi);
+    int jt = bj + cell_of(fth);
+    float at = frac_of(fth);
     float a0, a1, b0, b1, w;
     if (jt >= 1 && jt < poleRow) {
         // ordinary bilinear: no fold, no pole branch, both rows in range
@@ -236,13 +240,12 @@ struct Stage {
 template <int KIND, int SW>
 __device__ __forceinline__ float sample_st(const Stage& st, Rsrc f, const Grid& g, int bi, int lc,
                                            float fphi, int bj, int lr, float fth) {
-    const float kp = floorf(fphi), kt = floorf(fth);
-    const int lp = lc + (int)kp, lt = lr + (int)kt;
+    const int lp = lc + cell_of(fphi), lt = lr + cell_of(fth);
     const bool inside = ((unsigned)(lt - st.rlo[KIND]) < (unsigned)(st.rhi[KIND] - st.rlo[KIND])) &
                         ((unsigned)lp < (unsigned)st.cmax);
     if (inside) {
         const float* q = st.S + KIND * st.plane + __mul24(lt, SW) + lp;
-        const float ap = fphi - kp, at = fth - kt;
+        const float ap = frac_of(fphi), at = frac_of(fth);
         return lerpf(lerpf(q[0], q[1], ap), lerpf(q[SW], q[SW + 1], ap), at);
     }
     return sample<KIND>(f, g, bi, fphi, bj, fth);
@@ -274,12 +277,15 @@ __device__ __forceinline__ float advect_node_st(const Stage& st, Rsrc up, Rsrc u
 // displacement within the halo (see k_advect) and the tile is clear of the
 // pole rows, every sample is an ordinary bilinear read of the staged block,
 // with no per-sample check.  Same values, same weights as sample_st.
-template <int KIND, int SW>
-__device__ __forceinline__ float sample_fast(const float* __restrict__ S, int plane, int lc,
-                                             float fphi, int lr, float fth) {
-    const float kp = floorf(fphi), kt = floorf(fth);
-    const float* q = S + KIND * plane + __mul24(lr + (int)kt, SW) + lc + (int)kp;
-    const float ap = fphi - kp, at = fth - kt;
+template <int SW>
+__device__ __forceinline__ float sample_fast(const float* __restrict__ nb, float fphi, float fth) {
+    // nb: the field's staged value at the node; the stencil starts
+    // floor(fth) rows and floor(fphi) columns away, and both rows are read
+    // with ds_read2 at immediate offsets from one address
+    // the stencil offset floor(fth) SW + floor(fphi) formed in FP32 (exact:
+    // both terms are small integers) and converted once
+    const float* q = nb + (int)fmaf(floorf(fth), (float)SW, floorf(fphi));
+    const float ap = frac_of(fphi), at = frac_of(fth);
     return lerpf(lerpf(q[0], q[1], ap), lerpf(q[SW], q[SW + 1], ap), at);
 }
 
@@ -289,16 +295,18 @@ __device__ __forceinline__ float advect_node_fast(const float* __restrict__ S, i
     constexpr float pP = kNodeP[NODE] + kShiftP[K_VPHI], pT = kNodeT[NODE] + kShiftT[K_VPHI];
     constexpr float tP = kNodeP[NODE] + kShiftP[K_VTHETA], tT = kNodeT[NODE] + kShiftT[K_VTHETA];
     constexpr float sP = kNodeP[NODE] + kShiftP[SRC], sT = kNodeT[NODE] + kShiftT[SRC];
-    const int o = __mul24(lr, SW) + lc;
-    const float u0p = (NODE == K_VPHI) ? S[o] : sample_fast<K_VPHI, SW>(S, plane, lc, pP, lr, pT);
-    const float u0t = (NODE == K_VTHETA) ? S[plane + o] : sample_fast<K_VTHETA, SW>(S, plane, lc, tP, lr, tT);
+    const float* bp = S + __mul24(lr, SW) + lc;  // node in the u_phi plane
+    const float* bt = bp + plane;                // ... u_theta plane
+    const float* bs = bp + SRC * plane;          // ... source field plane
+    const float u0p = (NODE == K_VPHI) ? bp[0] : sample_fast<SW>(bp, pP, pT);
+    const float u0t = (NODE == K_VTHETA) ? bt[0] : sample_fast<SW>(bt, tP, tT);
     float dp = u0p * cofP;
     float dt = u0t * cofT;
-    const float ump = sample_fast<K_VPHI, SW>(S, plane, lc, pP - 0.5f * dp, lr, pT - 0.5f * dt);
-    const float umt = sample_fast<K_VTHETA, SW>(S, plane, lc, tP - 0.5f * dp, lr, tT - 0.5f * dt);
+    const float ump = sample_fast<SW>(bp, pP - 0.5f * dp, pT - 0.5f * dt);
+    const float umt = sample_fast<SW>(bt, tP - 0.5f * dp, tT - 0.5f * dt);
     dp = 0.5f * (u0p + ump) * cofP;
     dt = 0.5f * (u0t + umt) * cofT;
-    return sample_fast<SRC, SW>(S, plane, lc, sP - dp, lr, sT - dt);
+    return sample_fast<SW>(bs, sP - dp, sT - dt);
 }
 
 // Displacement budgets of the fast path, in cells.  Node columns sit at
@@ -366,18 +374,23 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         // max |u_phi|, |u_theta| of the staged block and max phi coefficient of
         // the tile's node rows (float bits)
         __shared__ unsigned umax[3];
+        // phi coefficients of node rows j0-1 .. j0+R (u_phi / rho nodes, u_theta nodes)
+        __shared__ float cofRow[2][kAdvMaxRows + 2];
         if (tid < 3) umax[tid] = 0u;
         __syncthreads();
         if (tid < R + 2) {  // node rows j0-1 .. j0+R, one per thread
             const int j = j0 - 1 + tid;
-            if (j >= 0 && j < nT) {
-                const float cm = fmaxf(cofPC[j], cofPT[j]);
-                atomicMax(&umax[2], __float_as_uint(cm != cm ? INFINITY : cm));
-            }
+            const bool in = j >= 0 && j < nT;
+            const float cc = in ? cofPC[j] : 0.f, ct = in ? cofPT[j] : 0.f;
+            cofRow[0][tid] = cc;
+            cofRow[1][tid] = ct;
+            const float cm = fmaxf(cc, ct);
+            atomicMax(&umax[2], __float_as_uint(cm != cm ? INFINITY : cm));
         }
         float mp = 0.f, mt = 0.f;
+        const unsigned mSC = (unsigned)((((uint64_t)1 << 32) + SC - 1) / SC);  // e / SC, e < 2^16
         for (int e = tid; e < SR * SC; e += nthr) {
-            const int r = e / SC, c = e - r * SC;
+            const int r = (int)__umulhi((unsigned)e, mSC), c = e - __mul24(r, SC);
             int gi = c0 - kEC + c;
             gi += (gi < 0) ? nP : 0;
             gi -= (gi >= nP) ? nP : 0;
@@ -454,16 +467,16 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         // no duplicated tail items (occupancy hides the LDS latency)
         auto advect_all = [&](auto F) {
             for (int idx = tid; idx < nU; idx += nthr) {
-                const int r = (int)__umulhi((unsigned)idx, mW2), c = idx - r * W2 - 1, j = uLo + r;
-                U[idx] = node(F, KPhi{}, c, j, cofPC[j]);
+                const int r = (int)__umulhi((unsigned)idx, mW2), c = idx - __mul24(r, W2) - 1, j = uLo + r;
+                U[idx] = node(F, KPhi{}, c, j, cofRow[0][j - j0 + 1]);
             }
             for (int idx = tid; idx < nV; idx += nthr) {
-                const int r = (int)__umulhi((unsigned)idx, mW1), c = idx - r * W1 - 1, j = vLo + r;
-                V[idx] = node(F, KTheta{}, c, j, cofPT[j]);
+                const int r = (int)__umulhi((unsigned)idx, mW1), c = idx - __mul24(r, W1) - 1, j = vLo + r;
+                V[idx] = node(F, KTheta{}, c, j, cofRow[1][j - j0 + 1]);
             }
             for (int idx = tid; idx < nR; idx += nthr) {
-                const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W, j = j0 + r;
-                const float a = node(F, KRho{}, c, j, cofPC[j]);
+                const int r = (int)__umulhi((unsigned)idx, mW), c = idx - __mul24(r, W), j = j0 + r;
+                const float a = node(F, KRho{}, c, j, cofRow[0][r + 1]);
                 orh[j * nP + c0 + c] = a;
                 flag_bad(s.status, !isfinite(a));
             }

@@ -10,7 +10,7 @@ namespace kam {
 // An advection tile of R rows and W <= 128 columns stages its input fields
 // over R + 2 kAdvHaloRows rows and W + 2 kAdvHaloCols columns of LDS, at a
 // row stride of adv_stride(W) floats (W rounded up to 64 or 128, plus halo).
-constexpr int kAdvHaloRows = 3, kAdvHaloCols = 4, kAdvMaxCols = 128;
+constexpr int kAdvHaloRows = 3, kAdvHaloCols = 4, kAdvMaxCols = 128, kAdvMaxRows = 64;
 constexpr int adv_stride(int W) { return (W <= 64 ? 64 : 128) + 2 * kAdvHaloCols; }
 
 hipError_t launch_advect(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,

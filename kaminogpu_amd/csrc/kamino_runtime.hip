@@ -136,7 +136,8 @@ int make_plan(int nT, int nP, Plan& p) {
 int adjust_plan(Plan& p, int advRows, int fftRows) {
     if (advRows > 0) {
         const int R = advRows > p.nT ? p.nT : advRows;
-        if (adv_lds(R, p.advCols, adv_stride(kAdvMaxCols), true) > 160 * 1024 - 256)
+        // the kernel's static LDS (row coefficients, reductions) takes under 1 KiB
+        if (R > kAdvMaxRows || adv_lds(R, p.advCols, adv_stride(kAdvMaxCols), true) > 159 * 1024)
             return set_err(KAM_EINVAL, "advection tile too tall");
         p.advRows = R;
         p.advLds = adv_lds(p.advRows, p.advCols, adv_stride(p.advCols), true);
