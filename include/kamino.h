@@ -27,7 +27,7 @@ enum {
     KAM_ENOMEM = -2,   /* device or host allocation failed */
     KAM_EHIP = -3,     /* a HIP runtime call failed */
     KAM_EAGAIN = -4,   /* engine has no slot / HBM budget for the request now */
-    KAM_ENOTFOUND = -5,/* unknown request id */
+    KAM_ENOTFOUND = -5,/* unknown request id, or an input file that cannot be read */
     KAM_ESTATE = -6,   /* call not valid in the object's current state */
     KAM_ENODEV = -7    /* no usable gfx950 device */
 };
@@ -164,6 +164,20 @@ int64_t kam_sim_particle_count(const kam_sim *sim);
  * draws the same four numbers per particle from SplitMix64(seed).  coords may
  * be NULL to query *count. */
 int kam_particles_seed(int n_theta, double density, uint64_t seed, float *coords, int64_t *count);
+
+/* Image inputs.  The reference reads them with OpenCV 2.4 (imread, flip around
+ * the vertical axis, resize to n_phi x n_theta, INTER_LINEAR); here images are
+ * PPM / PGM files (P6, P3, P5, P2) and the flip and bilinear resize restate
+ * OpenCV's 8-bit fixed-point algorithm.  KAM_ENOTFOUND when the file cannot
+ * be read.
+ * kam_image_density: rho[n_theta][n_phi] = (B + G + R) / 3 of the resized
+ * image over 255, as KaminoSolver::initDensityfromPic (KaminoSolver.cu:243-277).
+ * kam_image_colors: colors[3n] = (G, R, B) / 255 of the pixel under each
+ * particle's (phi, theta), the reference's colorBGR (KaminoParticles.cu:62-76);
+ * positions outside the image clamp to its edge. */
+int kam_image_density(const char *path, int n_theta, int n_phi, float *rho);
+int kam_image_colors(const char *path, int n_theta, int n_phi, const float *coords, int64_t n,
+                     float *colors);
 
 /* Bytes of device memory one simulation of this shape needs (28 per cell + 256). */
 size_t kam_sim_bytes(int n_theta, int n_phi);

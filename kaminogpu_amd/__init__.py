@@ -31,7 +31,7 @@ ABI_SYMBOLS = [
     "kam_sim_step", "kam_sim_step_phases", "kam_sim_get_spectrum", "kam_sim_sync", "kam_sim_status",
     "kam_sim_bytes", "kam_sim_benchmark", "kam_init_fields", "kam_engine_completed",
     "kam_sim_set_particles", "kam_sim_get_particles", "kam_sim_particle_count", "kam_particles_seed",
-    "kam_init_fbm", "kam_sim_poisson_residual",
+    "kam_init_fbm", "kam_sim_poisson_residual", "kam_image_density", "kam_image_colors",
     "kam_tune_set", "kam_tune_get", "kam_autotune", "kam_tune_load",
     "kam_engine_create", "kam_engine_destroy", "kam_engine_admit", "kam_engine_step",
     "kam_engine_poll", "kam_engine_fetch", "kam_engine_cancel", "kam_engine_sync",
@@ -93,6 +93,8 @@ def lib():
     L.kam_particles_seed.argtypes = [C.c_int, C.c_double, C.c_uint64, _fp, C.POINTER(C.c_int64)]
     L.kam_init_fbm.argtypes = [C.c_int, C.c_int, C.c_double, _fp, _fp]
     L.kam_sim_poisson_residual.argtypes = [C.c_void_p, _dpp, _dpp, _dpp]
+    L.kam_image_density.argtypes = [C.c_char_p, C.c_int, C.c_int, _fp]
+    L.kam_image_colors.argtypes = [C.c_char_p, C.c_int, C.c_int, _fp, C.c_int64, _fp]
     _bind_engine(L)
     _lib = L
     return L
@@ -166,6 +168,23 @@ def init_fbm(n_theta, radius=1.0):
     ut = np.empty((nt - 1, nph), np.float32)
     check(lib().kam_init_fbm(nt, nph, float(radius), _ptr(up), _ptr(ut)))
     return up, ut
+
+
+def image_density(path, n_theta):
+    """Density from a PPM/PGM image (kam_image_density): float32 [nTheta][nPhi]."""
+    nt = int(n_theta)
+    rho = np.empty((nt, 2 * nt), np.float32)
+    check(lib().kam_image_density(str(path).encode(), nt, 2 * nt, _ptr(rho)))
+    return rho
+
+
+def image_colors(path, n_theta, coords):
+    """Particle colours (G, R, B) / 255 from a PPM/PGM image (kam_image_colors)."""
+    c = np.ascontiguousarray(coords, dtype=np.float32).reshape(-1, 2)
+    out = np.empty((len(c), 3), np.float32)
+    check(lib().kam_image_colors(str(path).encode(), int(n_theta), 2 * int(n_theta),
+                                 _ptr(c) if len(c) else None, len(c), _ptr(out) if len(c) else None))
+    return out
 
 
 def particles_seed(n_theta, density, seed):

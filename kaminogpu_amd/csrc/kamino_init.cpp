@@ -14,8 +14,13 @@
 //     which the reference's divergence (KaminoCore.cu:587-638) maps to zero;
 //   * density: 0.5 + 0.1 U[0,1) per cell, or the band rho = 1 for
 //     |theta - pi/2| < 0.3 with a linear ramp over two cells.
+#include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
 #include <vector>
 
 // The FBM noise below is a hash of exact FP32 / FP64 operation sequences; no
@@ -257,5 +262,203 @@ extern "C" int kam_init_fbm(int n_theta, int n_phi, double radius, float* u_phi,
                 const float avg = (float)((du + dd) / 2.0);
                 u_theta[(j - 1) * n_phi + i] = avg * gain;
             }
+    return KAM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Image inputs (KaminoSolver::initDensityfromPic, KaminoSolver.cu:243-277, and
+// the particle colours of KaminoParticles.cu:8-18, 62-76).  The reference
+// reads images with OpenCV 2.4.13 (imread IMREAD_COLOR, flip around the
+// vertical axis, resize to nPhi x nTheta with the default INTER_LINEAR).
+// OpenCV is not part of this build: images are binary or ASCII PPM / PGM
+// (P6, P3, P5, P2; grey replicated to B = G = R), and flip and resize restate
+// OpenCV's scalar algorithm for 8-bit images: per axis the source coordinate
+// (d + 0.5) * scale - 0.5 in FP32, weights rounded to 11-bit fixed point
+// (columns clamped to the edge with weights (1, 0), rows clamped with their
+// fractional weights kept), a horizontal then a vertical pass, and the result
+// rounded as (sum + 2^21) >> 22.  OpenCV's SSE2 vertical pass pre-shifts by 4
+// bits and may differ from this in the last bit of a pixel.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Image {
+    int w = 0, h = 0;
+    std::vector<unsigned char> bgr;  // row-major, 3 bytes per pixel (B, G, R)
+};
+
+bool read_token(FILE* f, std::string& t) {
+    t.clear();
+    int c;
+    while ((c = std::fgetc(f)) != EOF) {
+        if (c == '#') {
+            while ((c = std::fgetc(f)) != EOF && c != '\n') {}
+            continue;
+        }
+        if (!std::isspace(c)) break;
+    }
+    if (c == EOF) return false;
+    do t.push_back((char)c);
+    while ((c = std::fgetc(f)) != EOF && !std::isspace(c));
+    return true;
+}
+
+bool load_pnm(const char* path, Image& im) {
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return false;
+    std::string magic, tw, th, tm;
+    bool ok = read_token(f, magic) && read_token(f, tw) && read_token(f, th) && read_token(f, tm);
+    const bool color = magic == "P6" || magic == "P3", binary = magic == "P6" || magic == "P5";
+    ok = ok && (color || magic == "P5" || magic == "P2");
+    if (ok) {
+        im.w = std::atoi(tw.c_str());
+        im.h = std::atoi(th.c_str());
+        const int maxv = std::atoi(tm.c_str());
+        ok = im.w > 0 && im.h > 0 && im.w <= 1 << 15 && im.h <= 1 << 15 && maxv > 0 && maxv < 256;
+        if (ok) {
+            const size_t n = (size_t)im.w * im.h, ch = color ? 3 : 1;
+            std::vector<unsigned char> raw(n * ch);
+            if (binary) {
+                ok = std::fread(raw.data(), 1, raw.size(), f) == raw.size();
+            } else {
+                std::string t;
+                for (size_t k = 0; k < raw.size() && ok; ++k) {
+                    ok = read_token(f, t);
+                    if (ok) raw[k] = (unsigned char)std::atoi(t.c_str());
+                }
+            }
+            im.bgr.resize(n * 3);
+            for (size_t p = 0; ok && p < n; ++p) {
+                const unsigned char r = raw[p * ch], g = raw[p * ch + (ch - 1) / 2], b = raw[p * ch + ch - 1];
+                // maxv < 255 is rescaled to 8 bits as an 8-bit decoder would
+                auto s8 = [&](unsigned char v) { return (unsigned char)((v * 255 + maxv / 2) / maxv); };
+                im.bgr[p * 3] = s8(b);
+                im.bgr[p * 3 + 1] = s8(g);
+                im.bgr[p * 3 + 2] = s8(r);
+            }
+        }
+    }
+    std::fclose(f);
+    return ok;
+}
+
+// cv::flip(src, dst, 1): mirror around the vertical axis
+void flip_columns(Image& im) {
+    for (int y = 0; y < im.h; ++y)
+        for (int x = 0; x < im.w / 2; ++x)
+            for (int c = 0; c < 3; ++c)
+                std::swap(im.bgr[((size_t)y * im.w + x) * 3 + c],
+                          im.bgr[((size_t)y * im.w + (im.w - 1 - x)) * 3 + c]);
+}
+
+// The horizontal axis of INTER_LINEAR: source column and the two 11-bit
+// weights per destination column, the coordinate clamped to the edge columns
+// with weight (1, 0) there.
+void linear_x(int ssize, int dsize, std::vector<int>& ofs, std::vector<short>& w0,
+              std::vector<short>& w1) {
+    const double scale = (double)ssize / dsize;
+    ofs.resize(dsize);
+    w0.resize(dsize);
+    w1.resize(dsize);
+    for (int d = 0; d < dsize; ++d) {
+        float fx = (float)((d + 0.5) * scale - 0.5);
+        int sx = (int)std::floor(fx);
+        fx -= (float)sx;
+        if (sx < 0) {
+            fx = 0.f;
+            sx = 0;
+        }
+        if (sx >= ssize - 1) {
+            fx = 0.f;
+            sx = ssize - 1;
+        }
+        ofs[d] = sx;
+        w0[d] = (short)std::lrint((1.f - fx) * 2048.f);
+        w1[d] = (short)std::lrint(fx * 2048.f);
+    }
+}
+
+// cv::resize(src, dst, Size(dw, dh)) with INTER_LINEAR, 8-bit, 3 channels.
+// Rows: the two source rows floor(fy) and floor(fy) + 1, each clamped to the
+// image, with the unclamped fraction's weights.
+Image resize_linear(const Image& s, int dw, int dh) {
+    Image d;
+    d.w = dw;
+    d.h = dh;
+    if (s.w == dw && s.h == dh) {
+        d.bgr = s.bgr;
+        return d;
+    }
+    d.bgr.resize((size_t)dw * dh * 3);
+    std::vector<int> xo;
+    std::vector<short> xa, xb;
+    linear_x(s.w, dw, xo, xa, xb);
+    const double scale_y = (double)s.h / dh;
+    std::vector<int> row0((size_t)dw * 3), row1((size_t)dw * 3);
+    auto hpass = [&](int sy, std::vector<int>& out) {
+        const unsigned char* S = &s.bgr[(size_t)sy * s.w * 3];
+        for (int x = 0; x < dw; ++x) {
+            const int sx = xo[x], sx1 = sx + 1 < s.w ? sx + 1 : sx;
+            for (int c = 0; c < 3; ++c)
+                out[(size_t)x * 3 + c] = S[sx * 3 + c] * xa[x] + S[sx1 * 3 + c] * xb[x];
+        }
+    };
+    auto clip = [&](int y) { return y < 0 ? 0 : (y >= s.h ? s.h - 1 : y); };
+    for (int y = 0; y < dh; ++y) {
+        float fy = (float)((y + 0.5) * scale_y - 0.5);
+        const int sy = (int)std::floor(fy);
+        fy -= (float)sy;
+        const int b0 = (short)std::lrint((1.f - fy) * 2048.f), b1 = (short)std::lrint(fy * 2048.f);
+        hpass(clip(sy), row0);
+        hpass(clip(sy + 1), row1);
+        unsigned char* D = &d.bgr[(size_t)y * dw * 3];
+        for (int k = 0; k < dw * 3; ++k) {
+            const int v = (row0[k] * b0 + row1[k] * b1 + (1 << 21)) >> 22;
+            D[k] = (unsigned char)(v < 0 ? 0 : (v > 255 ? 255 : v));
+        }
+    }
+    return d;
+}
+
+bool load_resized(const char* path, int n_theta, int n_phi, Image& out) {
+    Image im;
+    if (!path || !load_pnm(path, im)) return false;
+    flip_columns(im);
+    out = resize_linear(im, n_phi, n_theta);
+    return true;
+}
+
+}  // namespace
+
+extern "C" int kam_image_density(const char* path, int n_theta, int n_phi, float* rho) {
+    if (n_theta < 2 || n_phi < 2 || !rho) return KAM_EINVAL;
+    Image im;
+    if (!load_resized(path, n_theta, n_phi, im)) return KAM_ENOTFOUND;
+    // KaminoSolver.cu:263-273: (B + G + R) / 3 of the channels divided by 255
+    for (int j = 0; j < n_theta; ++j)
+        for (int i = 0; i < n_phi; ++i) {
+            const unsigned char* p = &im.bgr[((size_t)j * n_phi + i) * 3];
+            const float B = (float)(p[0] / 255.0), G = (float)(p[1] / 255.0), R = (float)(p[2] / 255.0);
+            rho[(size_t)j * n_phi + i] = (float)((double)(B + G + R) / 3.0);
+        }
+    return KAM_OK;
+}
+
+extern "C" int kam_image_colors(const char* path, int n_theta, int n_phi, const float* coords, int64_t n,
+                                float* colors) {
+    if (n_theta < 2 || n_phi < 2 || n < 0 || (n > 0 && (!coords || !colors))) return KAM_EINVAL;
+    Image im;
+    if (!load_resized(path, n_theta, n_phi, im)) return KAM_ENOTFOUND;
+    const float gridLen = (float)(6.28318530717958647692 / n_phi);
+    // KaminoParticles.cu:62-76: the pixel under the particle's seeded
+    // position, stored as (G, R, B) / 255 (the reference's colorBGR order)
+    for (int64_t k = 0; k < n; ++k) {
+        long x = (long)std::floor(coords[2 * k] / gridLen), y = (long)std::floor(coords[2 * k + 1] / gridLen);
+        x = x < 0 ? 0 : (x >= n_phi ? n_phi - 1 : x);
+        y = y < 0 ? 0 : (y >= n_theta ? n_theta - 1 : y);
+        const unsigned char* p = &im.bgr[((size_t)y * n_phi + x) * 3];
+        colors[3 * k] = (float)(p[1] / 255.0);
+        colors[3 * k + 1] = (float)(p[2] / 255.0);
+        colors[3 * k + 2] = (float)(p[0] / 255.0);
+    }
     return KAM_OK;
 }

@@ -160,8 +160,9 @@ __device__ __forceinline__ float advect_node(Rsrc up, Rsrc ut, Rsrc src, const G
 // G^2 x^3 + (1 + G vbar) x - ubar = 0, or ubar when |G| <= 1e-7.  The
 // reference evaluates Cardano's formula in FP32, whose two terms cancel to
 // the small root (relative error ~1e-3 near the equator); when 1 + G vbar > 0
-// the cubic is monotone and Newton from ubar / (1 + G vbar) converges to the
-// same unique root to FP32 precision.  Otherwise (three real roots possible)
+// the cubic is monotone and Newton from the smaller of the root's two bounds
+// converges to the same unique root to FP32 precision.  Otherwise (three real
+// roots possible)
 // the reference's own branch choice is evaluated in FP64.
 // ---------------------------------------------------------------------------
 __device__ double ref_root3_pos(double x) {
@@ -202,8 +203,12 @@ __device__ __forceinline__ float geo_root(float G, float vb, float ub) {
     if (!(fabsf(G) > 1e-7f)) return ub;
     const float b = fmaf(G, vb, 1.f);
     if (b > 0.f) {
+        // the root is bounded by both |ub| / b and cbrt(|ub| / G^2); Newton on
+        // the monotone, convex-on-each-side cubic from the smaller bound
+        // converges from above, quadratically within a few steps
         const float a = G * G;
         float x = ub * __builtin_amdgcn_rcpf(b);
+        if (a * x * x > b) x = copysignf(cbrtf(fabsf(ub) * __builtin_amdgcn_rcpf(a)), ub);
 #pragma unroll
         for (int k = 0; k < 5; ++k) {
             const float x2 = x * x;
@@ -697,12 +702,13 @@ __device__ __forceinline__ void wave_pass(const float2* __restrict__ src, float2
 }
 
 // Transforms the M complex values at `row` (LDS) in place; `scratch` is the
-// calling lane group's own M-entry LDS row, `lane` the lane within the group
-// of gs lanes.  Every lane of the wavefront calls it (rowless groups pass
+// calling lane group's own M-entry LDS row, `tw` the nPhi twiddles
+// exp(-2 pi i n / nPhi) staged in LDS, `lane` the lane within the group of gs
+// lanes.  Every lane of the wavefront calls it (rowless groups pass
 // active = false) so that the fences stay wavefront-uniform.
 template <int DIR>
-__device__ void wave_fft(float2* row, float2* scratch, const SimDesc& s, int lane, int gs, bool active) {
-    const float2* __restrict__ tw = reinterpret_cast<const float2*>(s.twN);
+__device__ void wave_fft(float2* row, float2* scratch, const float2* tw, const SimDesc& s, int lane,
+                         int gs, bool active) {
     const int M = s.M;
     float2* src = row;
     float2* dst = scratch;
@@ -756,8 +762,12 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
         const float* __restrict__ divS = s.rowf + RT_DIVS * nT;
         const float* __restrict__ divN = s.rowf + RT_DIVN * nT;
         const float* __restrict__ invHs = s.rowf + RT_INVHS * nT;
-        float2* zbuf = zlds;                                       // R rows of M
-        float2* scratch = zlds + (size_t)(R + grp) * M;            // this lane group's row
+        float2* twl = zlds;                                        // nPhi twiddles
+        float2* zbuf = zlds + nP;                                  // R rows of M
+        float2* scratch = zbuf + (size_t)(R + grp) * M;            // this lane group's row
+        const float2* __restrict__ twg = reinterpret_cast<const float2*>(s.twN);
+        for (int i = tid; i < nP; i += nthr) twl[i] = twg[i];
+        __syncthreads();
         const int nPass = (R + ngrp - 1) / ngrp;                   // uniform trip count
         for (int it = 0; it < nPass; ++it) {
             const int r = it * ngrp + grp;
@@ -783,11 +793,11 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
                 row[k] = make_float2(d0, d1);
             }
             wave_sync();
-            wave_fft<-1>(row, scratch, s, lane, gs, active);
+            wave_fft<-1>(row, scratch, twl, s, lane, gs, active);
         }
         __syncthreads();
         float2* __restrict__ spec = reinterpret_cast<float2*>(s.spec);
-        const float2* __restrict__ tw = reinterpret_cast<const float2*>(s.twN);
+        const float2* tw = twl;
         const float scale = 1.0f / (float)nP;
         const unsigned magicR = (unsigned)((((uint64_t)1 << 32) + R - 1) / R);
         for (int idx = tid; idx < R * M; idx += nthr) {
@@ -1094,10 +1104,13 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
         const int j0 = tile.row0, R = tile.rows;
         const int Rp = min(R + 1, nT - j0);  // rows of p needed
         const int gs = fft_group(M), grp = tid / gs, ngrp = nthr / gs, lane = tid - grp * gs;
-        float2* zbuf = zlds;                                        // Rp rows of M
-        float2* scratch = zlds + (size_t)(Rp + grp) * M;            // this lane group's row
+        float2* tw = zlds;                                          // nPhi twiddles
+        float2* zbuf = zlds + nP;                                   // Rp rows of M
+        float2* scratch = zbuf + (size_t)(Rp + grp) * M;            // this lane group's row
         const float2* __restrict__ spec = reinterpret_cast<const float2*>(s.spec);
-        const float2* __restrict__ tw = reinterpret_cast<const float2*>(s.twN);
+        const float2* __restrict__ twg = reinterpret_cast<const float2*>(s.twN);
+        for (int i = tid; i < nP; i += nthr) tw[i] = twg[i];
+        __syncthreads();
         const unsigned magicR = (unsigned)((((uint64_t)1 << 32) + Rp - 1) / Rp);
         for (int idx = tid; idx < Rp * M; idx += nthr) {
             const int n = (Rp == 1) ? idx : (int)__umulhi((unsigned)idx, magicR), r = idx - n * Rp, j = j0 + r;
@@ -1120,7 +1133,7 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
         const int nPass = (Rp + ngrp - 1) / ngrp;
         for (int it = 0; it < nPass; ++it) {
             const int r = it * ngrp + grp;
-            wave_fft<+1>(zbuf + (size_t)(r < Rp ? r : 0) * M, scratch, s, lane, gs, r < Rp);
+            wave_fft<+1>(zbuf + (size_t)(r < Rp ? r : 0) * M, scratch, tw, s, lane, gs, r < Rp);
         }
         __syncthreads();
         const float* p = reinterpret_cast<const float*>(zbuf);  // p[r][i], row stride nP

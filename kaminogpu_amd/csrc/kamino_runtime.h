@@ -94,6 +94,7 @@ private:
     std::vector<TriClass> tri_;
     size_t advLds_ = 0, fftLds_ = 0;
     int advStride_ = 0;  // staged row stride of the advection launch
+    int launch_tri(hipStream_t st, bool resid);
 };
 
 // Kernel-time categories reported by profiling.

@@ -66,9 +66,10 @@ static int fft_group_host(int M) {
     while (g < 64 && 4 * g < M) g *= 2;
     return g;
 }
-// LDS of an FFT tile: R + 1 rows of M complex plus the lane groups' scratch rows
+// LDS of an FFT tile: the nPhi = 2M twiddles, R + 1 rows of M complex and the
+// lane groups' scratch rows
 static size_t fft_lds(int R, int M) {
-    return (size_t)(R + 1 + 256 / fft_group_host(M)) * M * 2 * sizeof(float);
+    return (size_t)(2 + R + 1 + 256 / fft_group_host(M)) * M * 2 * sizeof(float);
 }
 
 // ---------------------------------------------------------------------------
@@ -109,7 +110,7 @@ int make_plan(int nT, int nP, Plan& p) {
     p.advCols = nP < kAdvMaxCols ? nP : kAdvMaxCols;
     p.advRows = nT < 8 ? nT : 8;
     // FFT tile: R rows (+1 halo row for the inverse, + scratch rows) in ~48 KiB
-    int Rf = (int)(49152 / (8 * (size_t)p.M)) - 1 - 256 / fft_group_host(p.M);
+    int Rf = (int)(49152 / (8 * (size_t)p.M)) - 3 - 256 / fft_group_host(p.M);
     Rf = Rf < 1 ? 1 : (Rf > 16 ? 16 : Rf);
     p.fftRows = Rf > nT ? nT : Rf;
     // tridiagonal: L lanes per mode, at most K rows per lane
@@ -257,6 +258,7 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
 // ---------------------------------------------------------------------------
 DeviceBatch::~DeviceBatch() {
     if (uploaded_) (void)hipEventSynchronize(uploaded_);
+
     if (dSims_) (void)hipFree(dSims_);
     if (dTiles_) (void)hipFree(dTiles_);
     if (hStage_) (void)hipHostFree(hStage_);
@@ -388,11 +390,16 @@ int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>
 
 static const int kGridCap = 1 << 20;
 
-int DeviceBatch::launch_tri_residual(hipStream_t st) {
+int DeviceBatch::launch_tri(hipStream_t st, bool resid) {
+    // one launch per (L, K) class, in stream order (side streams would let small
+    // classes overlap, but their fork/join graphs cost far more to capture each
+    // time a serving engine's membership changes)
     for (const TriClass& c : tri_)
-        KAM_HIP(launch_tridiag(c.L, c.K, dSims_, dTiles_ + c.off, c.count, kGridCap, st, true));
+        KAM_HIP(launch_tridiag(c.L, c.K, dSims_, dTiles_ + c.off, c.count, kGridCap, st, resid));
     return KAM_OK;
 }
+
+int DeviceBatch::launch_tri_residual(hipStream_t st) { return launch_tri(st, true); }
 
 int DeviceBatch::launch(int phases, int parity, hipStream_t st, int* newParity,
                         hipEvent_t* ev) {
@@ -415,8 +422,8 @@ int DeviceBatch::launch(int phases, int parity, hipStream_t st, int* newParity,
     }
     if (phases & KAM_PHASE_TRI) {
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_TRI], st));
-        for (const TriClass& c : tri_)
-            KAM_HIP(launch_tridiag(c.L, c.K, dSims_, dTiles_ + c.off, c.count, kGridCap, st));
+        int rc = launch_tri(st, false);
+        if (rc != KAM_OK) return rc;
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_TRI + 1], st));
     }
     if (phases & KAM_PHASE_IFFT) {

@@ -303,8 +303,8 @@ int main(int argc, char** argv) {
         std::cerr << "kamino: init must be fbm or sh\n";
         return 1;
     }
-    if (!c.densityImage.empty() || !c.colorImage.empty() || !c.solidImage.empty())
-        std::cerr << "kamino: image inputs are not supported; using the seeded initial state\n";
+    if (!c.solidImage.empty())
+        std::cerr << "kamino: the solid image is not used (nor is it by the reference's step)\n";
 
     const int nT = c.nTheta, nP = 2 * nT;
     kam_sim_desc d{};
@@ -334,6 +334,12 @@ int main(int argc, char** argv) {
         std::cerr << "kamino: bad initial-state parameters\n";
         return 2;
     }
+    // initDensityfromPic (KaminoSolver.cu:243-277) replaces the density
+    if (!c.kv.count("resume") && !c.densityImage.empty() &&
+        kam_image_density(c.densityImage.c_str(), nT, nP, rho.data()) != KAM_OK) {
+        std::cerr << "kamino: cannot read density image " << c.densityImage << " (PPM / PGM)\n";
+        return 2;
+    }
     if (kam_sim_set_fields(sim, up.data(), ut.data(), rho.data()) != KAM_OK) {
         std::cerr << "kamino: " << kam_last_error() << "\n";
         return 2;
@@ -360,6 +366,19 @@ int main(int argc, char** argv) {
             return 2;
         }
     }
+    // particle colours from the colour image (KaminoParticles.cu:62-76), else 0;
+    // write_particles_bgeo writes colorBGR[3i+1], [3i+2], [3i+3] for particle i
+    // (KaminoSolver.cu:382-384), reading one entry past the particle's own
+    std::vector<float> colorBGR;
+    if (particles) {
+        const size_t n = coords.size() / 2;
+        colorBGR.assign(3 * n + 1, 0.f);
+        if (!c.colorImage.empty() &&
+            kam_image_colors(c.colorImage.c_str(), nT, nP, coords.data(), (int64_t)n, colorBGR.data()) != KAM_OK) {
+            std::cerr << "kamino: cannot read colour image " << c.colorImage << " (PPM / PGM)\n";
+            return 2;
+        }
+    }
     std::vector<float> pos, vel, color;
     auto write_frame = [&](int frame) {
         if (kam_sim_get_fields(sim, up.data(), ut.data(), rho.data()) != KAM_OK) return false;
@@ -379,8 +398,9 @@ int main(int argc, char** argv) {
         if (format == "raw") return write_raw_particles(pfile, coords);
         const size_t n = coords.size() / 2;
         particle_points(c.radius, coords, pos);
-        vel.assign(3 * n, 0.f);    // write_particles_bgeo writes v = 0
-        color.assign(3 * n, 0.f);  // no colour image: colorBGR is 0
+        vel.assign(3 * n, 0.f);  // write_particles_bgeo writes v = 0
+        color.resize(3 * n);
+        for (size_t k = 0; k < 3 * n; ++k) color[k] = colorBGR[k + 1];
         return write_bgeo(pfile, (int)n, pos.data(), {{"v", 3, vel.data()}, {"color", 3, color.data()}});
     };
     if (!write_frame(0)) {

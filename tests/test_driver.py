@@ -117,6 +117,37 @@ def test_driver_default_init_is_the_reference_fbm(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_image_inputs(tmp_path):
+    """densityImage sets the initial density (initDensityfromPic); colorImage
+    colours the particles, written as colorBGR[3i+1..3i+3] like
+    write_particles_bgeo (KaminoSolver.cu:382-384)."""
+    nt = 16
+    rng = np.random.default_rng(4)
+    rgb = rng.integers(0, 256, (10, 20, 3), dtype=np.uint8)
+    (tmp_path / "d.ppm").write_bytes(b"P6\n20 10\n255\n" + rgb.tobytes())
+    write_cfg(tmp_path / "c.txt", [1.0, nt, 1, 0.001, 0.001, 1, 0, 1, 1, 1, 1,
+                                   str(tmp_path / "frame"), str(tmp_path / "part"),
+                                   str(tmp_path / "d.ppm"), "null", str(tmp_path / "d.ppm"),
+                                   "format=raw"])
+    r = run([str(tmp_path / "c.txt")], tmp_path)
+    assert r.returncode == 0, r.stderr
+    raw0 = np.frombuffer(open(tmp_path / "frame0.kamr", "rb").read()[28:], np.float32)
+    N = nt * 2 * nt
+    np.testing.assert_array_equal(raw0[2 * N - 2 * nt:].reshape(nt, 2 * nt), K.image_density(tmp_path / "d.ppm", nt))
+    write_cfg(tmp_path / "c2.txt", [1.0, nt, 1, 0.001, 0.001, 1, 0, 1, 1, 1, 1,
+                                    str(tmp_path / "frame"), str(tmp_path / "part"),
+                                    "null", "null", str(tmp_path / "d.ppm")])
+    r = run([str(tmp_path / "c2.txt")], tmp_path)
+    assert r.returncode == 0, r.stderr
+    parts = K.particles_seed(nt, 1, 1)
+    cols = K.image_colors(tmp_path / "d.ppm", nt, parts).reshape(-1)
+    want = np.concatenate([cols, [0.0]])[1:].reshape(-1, 3)
+    b = open(tmp_path / "part0.bgeo", "rb").read()
+    pts = _check_bgeo(b, len(parts), [("v", 3), ("color", 3)], radius=1.0)
+    np.testing.assert_array_equal(pts[:, 7:10], want)
+
+
+@pytest.mark.gpu
 def test_driver_bgeo_layout(tmp_path):
     nt = 16
     write_cfg(tmp_path / "c.txt", [2.0, nt, 200, 0.01, 0.02, 1, 0, 1, 1, 1, 1,
@@ -129,7 +160,7 @@ def test_driver_bgeo_layout(tmp_path):
     _check_bgeo(open(tmp_path / "part1.bgeo", "rb").read(), npart, [("v", 3), ("color", 3)])
 
 
-def _check_bgeo(b, n, attrs):
+def _check_bgeo(b, n, attrs, radius=2.0):
     assert b[:5] == b"BgeoV"
     ver, npts, nprim, _, _, npattr = struct.unpack(">6i", b[5:29])
     assert (ver, npts, nprim, npattr) == (5, n, 1, len(attrs))
@@ -144,7 +175,7 @@ def _check_bgeo(b, n, attrs):
     assert names == attrs
     width = 4 + sum(k for _, k in attrs)
     pts = np.frombuffer(b[off:off + npts * width * 4], ">f4").reshape(npts, width)
-    radius = np.linalg.norm(pts[:, :3], axis=1)
-    np.testing.assert_allclose(radius, 2.0, rtol=1e-6)  # points on the sphere
+    np.testing.assert_allclose(np.linalg.norm(pts[:, :3], axis=1), radius, rtol=1e-6)  # on the sphere
     np.testing.assert_allclose(pts[:, 3], 1.0)
     assert b[-2:] == b"\x00\xff"
+    return pts

@@ -42,6 +42,22 @@ def test_advection_parity(nt, geometric):
     assert max(errs) < TOL_FIELD
 
 
+@pytest.mark.parametrize("nt,dt", [(32, 0.005), (64, 0.002)])
+def test_advection_parity_fbm_velocity(nt, dt):
+    """The reference's own FBM initial velocity (|u| up to ~100, departures of
+    several cells, a cubic dominated by its G^2 x^3 term near the poles)."""
+    up, ut = K.init_fbm(nt)
+    _, _, rho = K.init_fields(nt, 8, 1, K.DENSITY_BAND)
+    state = f32(up, ut, rho)
+    log(f"[advect fbm] nt={nt} dt={dt} max|u|={np.abs(up).max():.1f}")
+    got = _gpu_phase(nt, dt, state, K.PHASE_ADVECT)
+    ref, masks, st = oracle_advect_geo(nt, dt, *state)
+    errs = [rel_l2(g, r, m) for g, r, m in zip(got, ref, masks)]
+    log(f"  rel L2 u_phi={errs[0]:.2e} u_theta={errs[1]:.2e} rho={errs[2]:.2e} "
+        f"ambiguous={[int(m.sum()) for m in masks]}")
+    assert max(errs) < TOL_FIELD
+
+
 @pytest.mark.parametrize("nt", [6, 8, 12, 32, 48, 64, 128, 160, 256])
 def test_projection_parity(nt):
     dt = 0.5 / nt
