@@ -209,12 +209,20 @@ __device__ __forceinline__ float geo_root(float G, float vb, float ub) {
         const float a = G * G;
         float x = ub * __builtin_amdgcn_rcpf(b);
         if (a * x * x > b) x = copysignf(cbrtf(fabsf(ub) * __builtin_amdgcn_rcpf(a)), ub);
-#pragma unroll
+        // at most five steps.  A lane stops updating once its step falls below
+        // FP32 resolution (so its result never depends on its neighbours), and
+        // the wavefront leaves when every lane has (most cells need one or two)
+        bool done = false;
         for (int k = 0; k < 5; ++k) {
             const float x2 = x * x;
             const float f = fmaf(a * x2, x, fmaf(b, x, -ub));
             const float fp = fmaf(3.f * a, x2, b);
-            x = fmaf(-f, __builtin_amdgcn_rcpf(fp), x);
+            const float dx = f * __builtin_amdgcn_rcpf(fp);
+            if (!done) {
+                x -= dx;
+                done = !(fabsf(dx) > 1e-7f * fabsf(x));
+            }
+            if (__all(done)) break;
         }
         return x;
     }
