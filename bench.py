@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven-steps", type=int, default=200, help="config-3 batch steps (0 skips)")
-    ap.add_argument("--serve-requests", type=int, default=2000, help="config-4 requests (0 skips)")
+    ap.add_argument("--serve-requests", type=int, default=10000, help="config-4 requests (0 skips)")
     ap.add_argument("--tune-cache", default=None,
                     help="tuning cache: sizes it holds are loaded, the others autotuned and appended")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,

@@ -27,11 +27,12 @@ def _gpu_phase(nt, dt, state, phases, geometric=True):
         return s.get_fields()
 
 
-@pytest.mark.parametrize("nt,geometric", [(32, True), (48, True), (64, True), (96, True),
-                                          (160, True), (256, True), (64, False), (256, False)])
+@pytest.mark.parametrize("nt,geometric", [(4, True), (6, True), (12, True), (32, True), (48, True),
+                                          (64, True), (96, True), (160, True), (256, True),
+                                          (8, False), (64, False), (256, False)])
 def test_advection_parity(nt, geometric):
     dt = 0.5 / nt
-    state = seeded_state(nt, L=min(16, nt // 4), seed=nt, dens=1, pre_steps=1)
+    state = seeded_state(nt, L=max(1, min(16, nt // 4)), seed=nt, dens=1, pre_steps=1)
     log(f"[advect] nt={nt} geometric={geometric} dt={dt}")
     got = _gpu_phase(nt, dt, state, K.PHASE_ADVECT, geometric)
     ref, masks, st = oracle_advect_geo(nt, dt, *state, geometric=geometric)
