@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven-steps", type=int, default=200, help="config-3 batch steps (0 skips)")
+    ap.add_argument("--large-steps", type=int, default=500, help="config-2 steps at 1024x2048 (0 skips)")
     ap.add_argument("--serve-requests", type=int, default=10000, help="config-4 requests (0 skips)")
     ap.add_argument("--tune-cache", default=None,
                     help="tuning cache: sizes it holds are loaded, the others autotuned and appended")
@@ -147,6 +148,22 @@ def run_served(K, n):
     return out
 
 
+def run_large(K, steps, nt=1024):
+    """Config 2: one 1024x2048 sphere, dt = 0.5/nTheta, seeded flow with L = 32,
+    stepped alone (plain launches, FP32 fields, FP64 tridiagonal)."""
+    dt = 0.5 / nt
+    with K.Sim(nt, dt) as s:
+        s.set_fields(*init_state(nt, 32, 2, 1))
+        s.benchmark(10)
+        total, per, _ = s.benchmark(steps)
+        _, _, kern = s.benchmark(10, per_kernel=True)
+    v = nt * 2 * nt * steps / (total * 1e-3)
+    log(f"large {nt}x{2 * nt}: {v:.3e} cell-steps/s, {total / steps:.4f} ms/step, kernel ms {np.round(kern, 4)}")
+    return {"n_theta": nt, "steps": steps, "cell_steps_per_s": v, "ms_per_step": total / steps,
+            "p99_step_ms": float(np.percentile(per, 99)),
+            "kernel_ms": dict(zip(KERNELS, [float(x) for x in kern]))}
+
+
 def measured_traffic(kernel, nt, batch):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary, scaled by the copy-kernel calibration; None when the
@@ -226,6 +243,7 @@ def main():
     log(f"batched x{B}: {batched:.3e} cell-steps/s, {ms_per_step:.4f} ms/step, "
         f"p50 {st.step_p50_ms:.4f} p99 {st.step_p99_ms:.4f}")
 
+    large = run_large(K, a.large_steps) if a.large_steps > 0 else None
     uneven = run_uneven(K, a.uneven_steps, a.tune_cache) if a.uneven_steps > 0 else None
     served = run_served(K, a.serve_requests) if a.serve_requests > 0 else None
 
@@ -255,6 +273,7 @@ def main():
         "plain": {"cell_steps_per_s": plain, "steps": a.plain_steps,
                   "p50_step_ms": float(np.percentile(p_steps, 50)),
                   "p99_step_ms": float(np.percentile(p_steps, 99))},
+        "large": large,
         "uneven_batch": uneven,
         "served": served,
         "kernel_ms": kern,
