@@ -22,6 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published peak
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9  # MI355X: 256 CUs, 64 VALU lanes per CU per clock, 2.4 GHz
 # algorithmic bytes per cell per launch (FP32 compulsory reads + writes)
 BYTES_PER_CELL = {"advect": 24.0, "div_fft": 12.0, "tridiag": 8.0, "ifft_grad": 20.0}
 KERNELS = ["advect", "div_fft", "tridiag", "ifft_grad"]
@@ -164,19 +165,19 @@ def run_large(K, steps, nt=1024):
             "kernel_ms": dict(zip(KERNELS, [float(x) for x in kern]))}
 
 
-def measured_traffic(kernel, nt, batch):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary, scaled by the copy-kernel calibration; None when the
-    summary is for another workload."""
+def profile_counters(kernel, nt, batch):
+    """Per-launch counters of `kernel` from the committed rocprofv3 summary of
+    this workload: HBM bytes (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
+    calibration) and VALU instructions; empty when the summary is for another
+    workload."""
     path = os.path.join(ROOT, "profiles", f"hbm_traffic_{batch}x{nt}x{2 * nt}.json")
     if not os.path.exists(path):
-        return None
+        return {}
     with open(path) as f:
         t = json.load(f)
     name = {"advect": "k_advect", "div_fft": "k_div_fft", "tridiag": "k_tridiag",
             "ifft_grad": "k_ifft_grad"}[kernel]
-    k = t["kernels"].get(name)
-    return None if k is None else k["hbm_bytes"]
+    return t["kernels"].get(name) or {}
 
 
 def copy_calibration(nbytes):
@@ -251,7 +252,12 @@ def main():
     dom = max(kern, key=kern.get)
     gbs = {k: BYTES_PER_CELL[k] * cells * B / (kern[k] * 1e-3) / 1e9 for k in KERNELS}
     achieved = gbs[dom]
-    traffic = measured_traffic(dom, nt, B)
+    counters = profile_counters(dom, nt, B)
+    traffic = counters.get("hbm_bytes")
+    # the same kernel against the VALU issue peak: wave instructions x 64 lanes
+    # per launch over the launch time, against 256 CUs x 64 lanes x 2.4 GHz
+    valu = counters.get("valu_insts")
+    valu_frac = None if valu is None else valu * 64 / (kern[dom] * 1e-3) / VALU_PEAK_LANE_OPS
     out = {
         "metric": METRIC,
         "value": batched,
@@ -280,7 +286,8 @@ def main():
         "kernel_gbs": gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": BYTES_PER_CELL[dom] * cells * B},
+                     "algorithmic_bytes": BYTES_PER_CELL[dom] * cells * B,
+                     "valu_issue_frac": valu_frac},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(nt, dt, a.cpu_seconds)
