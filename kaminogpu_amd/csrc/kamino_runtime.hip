@@ -105,13 +105,15 @@ int make_plan(int nT, int nP, Plan& p) {
     p.nfac = factor_m(p.M, p.fac);
     if (p.nfac < 0)
         return set_err(KAM_EINVAL, "n_theta must factor as 2^a 3^b 5^c (got %d)", nT);
-    // advection tile: 8 rows x 128 columns (about 33 KiB of LDS, so four
-    // workgroups fit a CU); narrower spheres use their full width
-    p.advCols = nP < kAdvMaxCols ? nP : kAdvMaxCols;
-    p.advRows = nT < 8 ? nT : 8;
-    // FFT tile: R rows (+1 halo row for the inverse, + scratch rows) in ~48 KiB
+    // advection tile: 16 rows x 64 columns (8 rows below nTheta = 128), about
+    // 30 KiB of LDS, the shape the autotuner keeps picking; narrower spheres
+    // use their full width.  A batch stages every tile at the widest member's
+    // row stride, so one common default width keeps mixed batches compact.
+    p.advCols = nP < 64 ? nP : 64;
+    p.advRows = nT < 128 ? (nT < 8 ? nT : 8) : 16;
+    // FFT tile: at most 8 rows (+1 halo row for the inverse, + scratch rows) in ~48 KiB
     int Rf = (int)(49152 / (8 * (size_t)p.M)) - 3 - 256 / fft_group_host(p.M);
-    Rf = Rf < 1 ? 1 : (Rf > 16 ? 16 : Rf);
+    Rf = Rf < 1 ? 1 : (Rf > 8 ? 8 : Rf);
     p.fftRows = Rf > nT ? nT : Rf;
     // tridiagonal: L lanes per mode, at most K rows per lane
     if (nT <= 64) { p.triL = 8; p.triK = 8; }
