@@ -855,25 +855,27 @@ __device__ __forceinline__ double shfl_grp(double v, int srcLaneInGroup, int lan
     return __shfl(v, laneBase + srcLaneInGroup, 64);
 }
 
-// RESID additionally writes, per mode, ||T x - d|| / ||d|| of the FP64
-// solution x before it is rounded to FP32 (to SimDesc::resid).
+// One tile of the tridiagonal solve: G = 256 / L modes of one simulation,
+// L lanes per mode.  The block's G mode columns and the FP64 row terms pass
+// through LDS (zs: G x KMAX x (L+1) float2; cs, ss: KMAX x (L+1) doubles):
+// global reads and writes stay contiguous, and lane l finds its row l K + k
+// at [k][l] (a padded row per k keeps the staging writes spread over the
+// banks).  RESID additionally writes, per mode, ||T x - d|| / ||d|| of the
+// FP64 solution x before it is rounded to FP32 (to SimDesc::resid).
 template <int L, int KMAX, bool RESID>
-__global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sims,
-                                                 const Tile* __restrict__ tiles, int ntiles) {
+__device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const Tile& tile,
+                                         float2* zsp, double* csp, double* ssp) {
     constexpr int G = 256 / L;  // modes per block
-    // The block's G mode columns and the FP64 row terms pass through LDS:
-    // global reads and writes stay contiguous, and lane l finds its row
-    // l K + k at [k][l] (a padded row per k keeps the staging writes spread
-    // over the banks).
-    __shared__ float2 zs[G][KMAX][L + 1];
-    __shared__ double cs[KMAX][L + 1], ss[KMAX][L + 1];
+    (void)G;
+    auto zs = [&](int gg, int k, int ll) -> float2& { return zsp[(gg * KMAX + k) * (L + 1) + ll]; };
+    auto cs = [&](int k, int ll) -> double& { return csp[k * (L + 1) + ll]; };
+    auto ss = [&](int k, int ll) -> double& { return ssp[k * (L + 1) + ll]; };
     const int tid = threadIdx.x;
     const int l = tid % L;               // lane within the mode group
     const int g = tid / L;               // mode within the block
     const int wl = threadIdx.x & 63;     // lane within the wavefront
     const int base = wl - l;             // first wavefront lane of this group
-    for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
-        const Tile tile = tiles[t];
+    {
         const SimDesc& s = sims[tile.sim];
         const int nT = s.nTheta;
         const bool active = g < tile.rows;
@@ -892,13 +894,13 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
         const double* __restrict__ isin2 = s.rowd + RD_INVSIN2 * nT;
         for (int i = tid; i < nT; i += 256) {
             const int li = (K == 1) ? i : (int)__umulhi((unsigned)i, magicK), ki = i - li * K;
-            cs[ki][li] = cot2h[i];
-            ss[ki][li] = isin2[i];
+            cs(ki, li) = cot2h[i];
+            ss(ki, li) = isin2[i];
         }
         for (int e = tid; e < tile.rows * nT; e += 256) {
             const int ge = (int)__umulhi((unsigned)e, magicT), i = e - ge * nT;
             const int li = (K == 1) ? i : (int)__umulhi((unsigned)i, magicK), ki = i - li * K;
-            zs[ge][ki][li] = blk[e];
+            zs(ge, ki, li) = blk[e];
         }
         __syncthreads();
         const double h2 = s.invH2;
@@ -917,12 +919,12 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
         for (int k = 0; k < KMAX; ++k) {
             if (k < cnt) {
                 const int i = r0 + k;
-                const float2 d = zs[g][k][l];
+                const float2 d = zs(g, k, l);
                 const double dre = (n == 0) ? 0.0 : (double)d.x;
                 const double dim = (double)d.y;
-                const double ct = cs[k][l];
+                const double ct = cs(k, l);
                 double a = h2 - ct, c = h2 + ct;
-                double b = -2.0 * h2 - nn * ss[k][l];
+                double b = -2.0 * h2 - nn * ss(k, l);
                 if (i == 0) {
                     b += a;
                     a = 0.0;
@@ -1037,7 +1039,7 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
                     xsr[k] = (n == 0) ? 0.0 : xr;
                     xsi[k] = xi;
                 } else {
-                    zs[g][k][l] = make_float2((n == 0) ? 0.f : (float)xr, (float)xi);
+                    zs(g, k, l) = make_float2((n == 0) ? 0.f : (float)xr, (float)xi);
                 }
             }
         }
@@ -1050,11 +1052,11 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
             for (int k = 0; k < KMAX; ++k) {
                 if (k < cnt) {
                     const int i = r0 + k;
-                    const float2 d = zs[g][k][l];
+                    const float2 d = zs(g, k, l);
                     const double dre = (n == 0) ? 0.0 : (double)d.x, dim = (double)d.y;
-                    const double ct = cs[k][l];
+                    const double ct = cs(k, l);
                     double a = h2 - ct, c = h2 + ct;
-                    double b = -2.0 * h2 - nn * ss[k][l];
+                    double b = -2.0 * h2 - nn * ss(k, l);
                     if (i == 0) {
                         b += a;
                         a = 0.0;
@@ -1080,13 +1082,42 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
             if (active && l == 0) s.resid[n] = dd > 0.0 ? sqrt(rr / dd) : 0.0;
 #pragma unroll
             for (int k = 0; k < KMAX; ++k)
-                if (k < cnt) zs[g][k][l] = make_float2((float)xsr[k], (float)xsi[k]);
+                if (k < cnt) zs(g, k, l) = make_float2((float)xsr[k], (float)xsi[k]);
         }
         __syncthreads();
         for (int e = tid; e < tile.rows * nT; e += 256) {
             const int ge = (int)__umulhi((unsigned)e, magicT), i = e - ge * nT;
             const int li = (K == 1) ? i : (int)__umulhi((unsigned)i, magicK), ki = i - li * K;
-            blk[e] = zs[ge][ki][li];
+            blk[e] = zs(ge, ki, li);
+        }
+    }
+}
+
+
+template <int L, int KMAX, bool RESID>
+__global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sims,
+                                                 const Tile* __restrict__ tiles, int ntiles) {
+    __shared__ float2 zs[(256 / L) * KMAX * (L + 1)];
+    __shared__ double cs[KMAX * (L + 1)], ss[KMAX * (L + 1)];
+    for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) tri_tile<L, KMAX, RESID>(sims, tiles[t], zs, cs, ss);
+}
+
+// Every class with K = 8 rows per lane in one launch: a tile carries its lanes
+// per mode in col0, and the block takes that variant (mixed-size batches
+// would otherwise queue one small launch per class).
+template <bool RESID>
+__global__ __launch_bounds__(256) void k_tridiag_mixed(const SimDesc* __restrict__ sims,
+                                                       const Tile* __restrict__ tiles, int ntiles) {
+    constexpr int K = 8, ZS = 32 * K * 9, CS = K * 65;  // largest over L = 8 ... 64
+    __shared__ float2 zs[ZS];
+    __shared__ double cs[CS], ss[CS];
+    for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
+        const Tile tile = tiles[t];
+        switch (tile.col0) {
+            case 8: tri_tile<8, K, RESID>(sims, tile, zs, cs, ss); break;
+            case 16: tri_tile<16, K, RESID>(sims, tile, zs, cs, ss); break;
+            case 32: tri_tile<32, K, RESID>(sims, tile, zs, cs, ss); break;
+            default: tri_tile<64, K, RESID>(sims, tile, zs, cs, ss); break;
         }
     }
 }
@@ -1211,6 +1242,13 @@ hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const
                           int ntiles, int gridCap, hipStream_t st, bool resid) {
     if (ntiles <= 0) return hipSuccess;
     const dim3 grid(grid_for(ntiles, gridCap)), block(256);
+    if (lanesPerMode == 0 && kmax == 8) {
+        if (resid)
+            hipLaunchKernelGGL((k_tridiag_mixed<true>), grid, block, 0, st, sims, tiles, ntiles);
+        else
+            hipLaunchKernelGGL((k_tridiag_mixed<false>), grid, block, 0, st, sims, tiles, ntiles);
+        return hipGetLastError();
+    }
 #define KAM_TRI(LL, KK)                                                                        \
     if (lanesPerMode == LL && kmax == KK) {                                                    \
         if (resid)                                                                             \

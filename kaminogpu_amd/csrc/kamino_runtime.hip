@@ -351,19 +351,35 @@ int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>
         fftLds_ = std::max(fftLds_, fft_lds(fftRows[s], p.M));
     }
     nFft_ = (int)hTiles_.size() - fftOff_;
-    // tridiagonal tiles, grouped by kernel class
+    // tridiagonal tiles, grouped by kernel class; when several classes of
+    // K = 8 rows per lane are present they share one mixed launch (L = 0),
+    // each tile carrying its lanes per mode in col0
     const int classes[5][2] = {{8, 8}, {16, 8}, {32, 8}, {64, 8}, {64, 16}};
+    int k8 = 0;
+    for (auto& c : classes)
+        if (c[1] == 8)
+            for (const Plan& p : plans)
+                if (p.triL == c[0] && p.triK == 8) {
+                    ++k8;
+                    break;
+                }
+    const bool mixed = k8 > 1;
     for (auto& c : classes) {
+        const bool inMixed = mixed && c[1] == 8;
         const int off = (int)hTiles_.size();
         for (size_t s = 0; s < sims.size(); ++s) {
             const Plan& p = plans[s];
             if (p.triL != c[0] || p.triK != c[1]) continue;
             const int per = 256 / p.triL;
             for (int n = 0; n < p.M; n += per)
-                hTiles_.push_back(Tile{(int)s, n, std::min(per, p.M - n), 0});
+                hTiles_.push_back(Tile{(int)s, n, std::min(per, p.M - n), inMixed ? p.triL : 0});
         }
         const int cnt = (int)hTiles_.size() - off;
-        if (cnt > 0) tri_.push_back(TriClass{c[0], c[1], off, cnt});
+        if (cnt == 0) continue;
+        if (inMixed && !tri_.empty() && tri_.back().L == 0)
+            tri_.back().count += cnt;  // classes are contiguous in the tile list
+        else
+            tri_.push_back(TriClass{inMixed ? 0 : c[0], c[1], off, cnt});
     }
     // particle tiles: runs of kPartTile particles
     constexpr int kPartTile = 2048;
