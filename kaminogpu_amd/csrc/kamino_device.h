@@ -59,7 +59,7 @@ struct SimDesc {
     unsigned magicP;       // ceil(2^32 / nPhi): i / nPhi == umulhi(i, magicP) for i < 2^16
     float invNP;           // 1 / nPhi
     int advCols;           // columns per advection tile (W <= min(nPhi, 255))
-    unsigned magicW[3];    // ceil(2^32 / (W + k)), k = 0, 1, 2
+    unsigned magicW[5];    // ceil(2^32 / d) for d = W, W + 1, W + 2, the staged width W + 8 and its quarter
     int* status;           // watchdog word: bit 0 set when a step produced a non-finite value
     float* part[2];        // particles, float2 (phi, theta) radians [nPart], per parity
     int nPart;             // particle count (0: none)

@@ -212,11 +212,7 @@ int rebuild(kam_engine* e) {
             r.plan = fresh;
             r.desc.advRows = fresh.advRows;
             r.desc.fftRows = fresh.fftRows;
-            r.desc.advCols = fresh.advCols;
-            for (int k = 0; k < 3; ++k) {
-                const uint64_t w = (uint64_t)fresh.advCols + k;
-                r.desc.magicW[k] = (unsigned)((((uint64_t)1 << 32) + w - 1) / w);
-            }
+            set_tile_cols(r.desc, fresh.advCols);
         }
         sims.push_back(r.desc);
         plans.push_back(r.plan);

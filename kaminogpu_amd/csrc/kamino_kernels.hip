@@ -37,6 +37,9 @@ __device__ __forceinline__ Rsrc make_rsrc(const float* p, int nfloats) {
 __device__ __forceinline__ float ldf(Rsrc r, int elem) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, elem * 4, 0, 0));
 }
+__device__ __forceinline__ float4 ldf4(Rsrc r, int elem) {
+    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 4, 0, 0));
+}
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
@@ -133,26 +136,25 @@ __device__ constexpr float kShiftP[3] = {0.5f, 0.f, 0.f};
 __device__ constexpr float kShiftT[3] = {-0.5f, -1.f, -0.5f};
 
 // RK2 backtrace from node (i, j) of grid NODE, sampling field `src` of grid
-// SRC at the departure point (KaminoCore.cu:186-319).  cofP converts u_phi to
-// phi cells per step at this node's latitude, cofT u_theta to theta cells.
-// A field sampled at its own node is the node value (both offsets integral,
-// the ordinary branch with zero weights), so that read is direct.
+// SRC at the departure point (KaminoCore.cu:186-319).  hP is half the factor
+// that converts u_phi to phi cells per step at this node's latitude, hT half
+// the one for u_theta to theta cells: the midpoint lies at x - h u0 and the
+// departure point at x - h (u0 + u_mid), one fma per coordinate (every
+// advection path uses exactly these roundings).  A field sampled at its own
+// node is the node value (both offsets integral, the ordinary branch with
+// zero weights), so that read is direct.
 template <int NODE, int SRC>
 __device__ __forceinline__ float advect_node(Rsrc up, Rsrc ut, Rsrc src, const Grid& g, int i, int j,
-                                             float cofP, float cofT) {
+                                             float hP, float hT) {
     constexpr float pP = kNodeP[NODE] + kShiftP[K_VPHI], pT = kNodeT[NODE] + kShiftT[K_VPHI];
     constexpr float tP = kNodeP[NODE] + kShiftP[K_VTHETA], tT = kNodeT[NODE] + kShiftT[K_VTHETA];
     constexpr float sP = kNodeP[NODE] + kShiftP[SRC], sT = kNodeT[NODE] + kShiftT[SRC];
     const int o = __mul24(j, g.nP) + i;
     const float u0p = (NODE == K_VPHI) ? ldf(up, o) : sample<K_VPHI>(up, g, i, pP, j, pT);
     const float u0t = (NODE == K_VTHETA) ? ldf(ut, o) : sample<K_VTHETA>(ut, g, i, tP, j, tT);
-    float dp = u0p * cofP;
-    float dt = u0t * cofT;
-    const float ump = sample<K_VPHI>(up, g, i, pP - 0.5f * dp, j, pT - 0.5f * dt);
-    const float umt = sample<K_VTHETA>(ut, g, i, tP - 0.5f * dp, j, tT - 0.5f * dt);
-    dp = 0.5f * (u0p + ump) * cofP;
-    dt = 0.5f * (u0t + umt) * cofT;
-    return sample<SRC>(src, g, i, sP - dp, j, sT - dt);
+    const float ump = sample<K_VPHI>(up, g, i, fmaf(-u0p, hP, pP), j, fmaf(-u0t, hT, pT));
+    const float umt = sample<K_VTHETA>(ut, g, i, fmaf(-u0p, hP, tP), j, fmaf(-u0t, hT, tT));
+    return sample<SRC>(src, g, i, fmaf(-(u0p + ump), hP, sP), j, fmaf(-(u0t + umt), hT, sT));
 }
 
 // ---------------------------------------------------------------------------
@@ -269,7 +271,7 @@ __device__ __forceinline__ float sample_st(const Stage& st, Rsrc f, const Grid& 
 template <int NODE, int SRC, int SW>
 __device__ __forceinline__ float advect_node_st(const Stage& st, Rsrc up, Rsrc ut, Rsrc src,
                                                 const Grid& g, int i, int j, int lc, int lr,
-                                                float cofP, float cofT) {
+                                                float hP, float hT) {
     constexpr float pP = kNodeP[NODE] + kShiftP[K_VPHI], pT = kNodeT[NODE] + kShiftT[K_VPHI];
     constexpr float tP = kNodeP[NODE] + kShiftP[K_VTHETA], tT = kNodeT[NODE] + kShiftT[K_VTHETA];
     constexpr float sP = kNodeP[NODE] + kShiftP[SRC], sT = kNodeT[NODE] + kShiftT[SRC];
@@ -277,13 +279,27 @@ __device__ __forceinline__ float advect_node_st(const Stage& st, Rsrc up, Rsrc u
     const float u0p = (NODE == K_VPHI) ? st.S[o] : sample_st<K_VPHI, SW>(st, up, g, i, lc, pP, j, lr, pT);
     const float u0t = (NODE == K_VTHETA) ? st.S[st.plane + o]
                                          : sample_st<K_VTHETA, SW>(st, ut, g, i, lc, tP, j, lr, tT);
-    float dp = u0p * cofP;
-    float dt = u0t * cofT;
-    const float ump = sample_st<K_VPHI, SW>(st, up, g, i, lc, pP - 0.5f * dp, j, lr, pT - 0.5f * dt);
-    const float umt = sample_st<K_VTHETA, SW>(st, ut, g, i, lc, tP - 0.5f * dp, j, lr, tT - 0.5f * dt);
-    dp = 0.5f * (u0p + ump) * cofP;
-    dt = 0.5f * (u0t + umt) * cofT;
-    return sample_st<SRC, SW>(st, src, g, i, lc, sP - dp, j, lr, sT - dt);
+    const float ump = sample_st<K_VPHI, SW>(st, up, g, i, lc, fmaf(-u0p, hP, pP), j, lr, fmaf(-u0t, hT, pT));
+    const float umt =
+        sample_st<K_VTHETA, SW>(st, ut, g, i, lc, fmaf(-u0p, hP, tP), j, lr, fmaf(-u0t, hT, tT));
+    return sample_st<SRC, SW>(st, src, g, i, lc, fmaf(-(u0p + ump), hP, sP), j, lr,
+                              fmaf(-(u0t + umt), hT, sT));
+}
+
+// An opaque copy of v, which keeps the compiler from re-associating an
+// address: the node's staged offset stays in one register and each sample
+// adds its stencil offset to it with one shift-add.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// (int)floorf(x) in one instruction (the compiler selects it only under
+// no-NaN math); x is a departure coordinate, |x| < 2^24.
+__device__ __forceinline__ int floor_i(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
 }
 
 // Tile-wide fast path: when the staged velocities bound every RK2
@@ -295,31 +311,48 @@ __device__ __forceinline__ float sample_fast(const float* __restrict__ nb, float
     // nb: the field's staged value at the node; the stencil starts
     // floor(fth) rows and floor(fphi) columns away, and both rows are read
     // with ds_read2 at immediate offsets from one address
-    // the stencil offset floor(fth) SW + floor(fphi) formed in FP32 (exact:
-    // both terms are small integers) and converted once
-    const float* q = nb + (int)fmaf(floorf(fth), (float)SW, floorf(fphi));
+    // one mad for the stencil offset, one shift-add for its address
+    const float* q = nb + opaque(__mul24(floor_i(fth), SW) + floor_i(fphi));
     const float ap = frac_of(fphi), at = frac_of(fth);
     return lerpf(lerpf(q[0], q[1], ap), lerpf(q[SW], q[SW + 1], ap), at);
 }
 
+// A sample at a constant offset (fphi, fth) from the node: both weights are
+// 0 or 1/2, and a zero weight reads one value (a + 0 (b - a) is a for the
+// finite velocities of a fast tile).
+constexpr int cfloor(float x) { return (x >= 0.f || (float)(int)x == x) ? (int)x : (int)x - 1; }
+
+template <int SW, int IP, int IT, bool HP, bool HT>
+__device__ __forceinline__ float sample_half(const float* __restrict__ nb) {
+    const float* q = nb + IT * SW + IP;
+    auto row = [](const float* r) { return HP ? lerpf(r[0], r[1], 0.5f) : r[0]; };
+    return HT ? lerpf(row(q), row(q + SW), 0.5f) : row(q);
+}
+
+template <int SW, int NODE, int KIND>
+__device__ __forceinline__ float sample_node_const(const float* __restrict__ nb) {
+    constexpr float fp = kNodeP[NODE] + kShiftP[KIND], ft = kNodeT[NODE] + kShiftT[KIND];
+    constexpr int ip = cfloor(fp), it = cfloor(ft);
+    static_assert((fp - ip == 0.f || fp - ip == 0.5f) && (ft - it == 0.f || ft - it == 0.5f),
+                  "node offsets are whole or half cells");
+    return sample_half<SW, ip, it, (fp - ip) != 0.f, (ft - it) != 0.f>(nb);
+}
+
 template <int NODE, int SRC, int SW>
 __device__ __forceinline__ float advect_node_fast(const float* __restrict__ S, int plane, int lc,
-                                                  int lr, float cofP, float cofT) {
+                                                  int lr, float hP, float hT) {
     constexpr float pP = kNodeP[NODE] + kShiftP[K_VPHI], pT = kNodeT[NODE] + kShiftT[K_VPHI];
     constexpr float tP = kNodeP[NODE] + kShiftP[K_VTHETA], tT = kNodeT[NODE] + kShiftT[K_VTHETA];
     constexpr float sP = kNodeP[NODE] + kShiftP[SRC], sT = kNodeT[NODE] + kShiftT[SRC];
-    const float* bp = S + __mul24(lr, SW) + lc;  // node in the u_phi plane
-    const float* bt = bp + plane;                // ... u_theta plane
-    const float* bs = bp + SRC * plane;          // ... source field plane
-    const float u0p = (NODE == K_VPHI) ? bp[0] : sample_fast<SW>(bp, pP, pT);
-    const float u0t = (NODE == K_VTHETA) ? bt[0] : sample_fast<SW>(bt, tP, tT);
-    float dp = u0p * cofP;
-    float dt = u0t * cofT;
-    const float ump = sample_fast<SW>(bp, pP - 0.5f * dp, pT - 0.5f * dt);
-    const float umt = sample_fast<SW>(bt, tP - 0.5f * dp, tT - 0.5f * dt);
-    dp = 0.5f * (u0p + ump) * cofP;
-    dt = 0.5f * (u0t + umt) * cofT;
-    return sample_fast<SW>(bs, sP - dp, sT - dt);
+    const int o = opaque(__mul24(lr, SW) + lc);
+    const float* bp = S + o;                // node in the u_phi plane
+    const float* bt = S + (o + plane);      // ... u_theta plane
+    const float* bs = S + (o + SRC * plane);  // ... source field plane
+    const float u0p = (NODE == K_VPHI) ? bp[0] : sample_node_const<SW, NODE, K_VPHI>(bp);
+    const float u0t = (NODE == K_VTHETA) ? bt[0] : sample_node_const<SW, NODE, K_VTHETA>(bt);
+    const float ump = sample_fast<SW>(bp, fmaf(-u0p, hP, pP), fmaf(-u0t, hT, pT));
+    const float umt = sample_fast<SW>(bt, fmaf(-u0p, hP, tP), fmaf(-u0t, hT, tT));
+    return sample_fast<SW>(bs, fmaf(-(u0p + ump), hP, sP), fmaf(-(u0t + umt), hT, sT));
 }
 
 // Displacement budgets of the fast path, in cells.  Node columns sit at
@@ -332,7 +365,8 @@ constexpr float kFastDPhi = 2.4f, kFastDTheta = 1.4f;
 // ---------------------------------------------------------------------------
 // k_advect: one tile = rows [j0, j0 + R) x columns [c0, c0 + W) of one
 // simulation, W <= SW - 2 kEC.  The tile first stages its input block (see
-// Stage).  With the geometric update on, it then advects into LDS the u_phi
+// Stage).  With the geometric update on, it advects the rho nodes first
+// (straight to HBM) and then, over the dead staged rho plane, into LDS the u_phi
 // values of rows j0..j0+R and columns c0-1..c0+W, and the u_theta values of
 // rows j0-1..j0+R and columns c0-1..c0+W-1 (the update reads one row either
 // side and one column to the west; the restaggered u_phi one more column to
@@ -346,7 +380,7 @@ template <int SW>
 __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims,
                                                 const Tile* __restrict__ tiles, int ntiles,
                                                 int parity) {
-    extern __shared__ float lds[];
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
@@ -362,7 +396,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         float* __restrict__ orh = s.state[parity ^ 1][F_RHO];
         const float* __restrict__ cofPC = s.rowf + RT_COFPHI_C * nT;
         const float* __restrict__ cofPT = s.rowf + RT_COFPHI_T * nT;
-        const float cofT = s.cofTheta;
+        const float hT = 0.5f * s.cofTheta;  // half of theta cells per unit u_theta
         const int j0 = tile.row0, R = tile.rows, c0 = tile.col0;
         const int W = s.advCols, W1 = W + 1, W2 = W + 2;
         const unsigned mW = s.magicW[0], mW1 = s.magicW[1], mW2 = s.magicW[2];
@@ -384,62 +418,86 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         st.rlo[K_VPHI] = st.rlo[K_VTHETA] = st.rlo[K_CENTERED] = max(0, 1 - js);
         st.rhi[K_VPHI] = st.rhi[K_CENTERED] = min(SR - 1, nT - 1 - js);
         st.rhi[K_VTHETA] = min(SR - 1, nT - 2 - js);
-        // max |u_phi|, |u_theta| of the staged block and max phi coefficient of
-        // the tile's node rows (float bits)
+        // max |u_phi|, |u_theta| of the staged block and max half phi
+        // coefficient of the tile's node rows (float bits)
         __shared__ unsigned umax[3];
-        // phi coefficients of node rows j0-1 .. j0+R (u_phi / rho nodes, u_theta nodes)
+        // half phi coefficients of node rows j0-1 .. j0+R (u_phi / rho nodes, u_theta nodes)
         __shared__ float cofRow[2][kAdvMaxRows + 2];
         if (tid < 3) umax[tid] = 0u;
         __syncthreads();
         if (tid < R + 2) {  // node rows j0-1 .. j0+R, one per thread
             const int j = j0 - 1 + tid;
             const bool in = j >= 0 && j < nT;
-            const float cc = in ? cofPC[j] : 0.f, ct = in ? cofPT[j] : 0.f;
+            const float cc = in ? 0.5f * cofPC[j] : 0.f, ct = in ? 0.5f * cofPT[j] : 0.f;
             cofRow[0][tid] = cc;
             cofRow[1][tid] = ct;
             const float cm = fmaxf(cc, ct);
             atomicMax(&umax[2], __float_as_uint(cm != cm ? INFINITY : cm));
         }
-        float mp = 0.f, mt = 0.f;
-        const unsigned mSC = (unsigned)((((uint64_t)1 << 32) + SC - 1) / SC);  // e / SC, e < 2^16
-        for (int e = tid; e < SR * SC; e += nthr) {
-            const int r = (int)__umulhi((unsigned)e, mSC), c = e - __mul24(r, SC);
-            int gi = c0 - kEC + c;
-            gi += (gi < 0) ? nP : 0;
-            gi -= (gi >= nP) ? nP : 0;
-            // rows outside the field read as 0 (buffer range check) and are never sampled
-            const int o = (js + r) * nP + gi, l = r * SW + c;
-            const float a = ldf(up, o), b = ldf(ut, o);
-            lds[l] = a;
-            lds[plane + l] = b;
-            lds[2 * plane + l] = ldf(rh, o);
-            // fmaxf drops NaN; a non-finite value must force the checked path
-            mp = (fabsf(a) <= mp) ? mp : fabsf(a);
-            mt = (fabsf(b) <= mt) ? mt : fabsf(b);
+        // running max of |u| as bits: non-negative floats (and NaN / Inf,
+        // above every finite value) order as unsigned, so a non-finite value
+        // forces the checked path (fmaxf would drop a NaN)
+        unsigned mp = 0u, mt = 0u;
+        auto absbits = [](float v) { return __float_as_uint(v) & 0x7fffffffu; };
+        if (((nP | W) & 3) == 0) {
+            // four columns per item: c0 - kEC, SC, nP and SW are multiples of
+            // 4, so a float4 never straddles the phi seam and lands 16-byte
+            // aligned in LDS
+            const int SC4 = SC >> 2;
+            const unsigned mSC4 = s.magicW[4];  // e / SC4, e < 2^16
+            for (int e = tid; e < SR * SC4; e += nthr) {
+                const int r = (int)__umulhi((unsigned)e, mSC4), c = (e - __mul24(r, SC4)) << 2;
+                int gi = c0 - kEC + c;
+                gi += (gi < 0) ? nP : 0;
+                gi -= (gi >= nP) ? nP : 0;
+                // rows outside the field read as 0 (buffer range check) and are never sampled
+                const int o = (js + r) * nP + gi, l = r * SW + c;
+                const float4 a = ldf4(up, o), b = ldf4(ut, o);
+                *reinterpret_cast<float4*>(lds + l) = a;
+                *reinterpret_cast<float4*>(lds + plane + l) = b;
+                *reinterpret_cast<float4*>(lds + 2 * plane + l) = ldf4(rh, o);
+                mp = max(mp, max(max(absbits(a.x), absbits(a.y)), max(absbits(a.z), absbits(a.w))));
+                mt = max(mt, max(max(absbits(b.x), absbits(b.y)), max(absbits(b.z), absbits(b.w))));
+            }
+        } else {
+            const unsigned mSC = s.magicW[3];  // e / SC, e < 2^16
+            for (int e = tid; e < SR * SC; e += nthr) {
+                const int r = (int)__umulhi((unsigned)e, mSC), c = e - __mul24(r, SC);
+                int gi = c0 - kEC + c;
+                gi += (gi < 0) ? nP : 0;
+                gi -= (gi >= nP) ? nP : 0;
+                const int o = (js + r) * nP + gi, l = r * SW + c;
+                const float a = ldf(up, o), b = ldf(ut, o);
+                lds[l] = a;
+                lds[plane + l] = b;
+                lds[2 * plane + l] = ldf(rh, o);
+                mp = max(mp, absbits(a));
+                mt = max(mt, absbits(b));
+            }
         }
-        // non-negative floats (and NaN / Inf, above every finite value) order as unsigned
-        atomicMax(&umax[0], __float_as_uint(mp));
-        atomicMax(&umax[1], __float_as_uint(mt));
+        atomicMax(&umax[0], mp);
+        atomicMax(&umax[1], mt);
         __syncthreads();
         // fast tile: every sample an ordinary bilinear read of the staged block
         bool fast;
         {
             const float Up = __uint_as_float(umax[0]), Ut = __uint_as_float(umax[1]);
-            const float cmax = __uint_as_float(umax[2]);
-            fast = (j0 >= kER + 1) && (j0 + R <= nT - kER - 1) && (Up * cmax <= kFastDPhi) &&
-                   (Ut * cofT <= kFastDTheta);
+            const float hmax = __uint_as_float(umax[2]);
+            // a departure moves at most 2 h max|u| cells
+            fast = (j0 >= kER + 1) && (j0 + R <= nT - kER - 1) && (Up * hmax <= 0.5f * kFastDPhi) &&
+                   (Ut * hT <= 0.5f * kFastDTheta);
         }
 
         // one backtrace at node (c, j) of the tile (c from -1, j global): the
         // branch-free staged sampler on fast tiles, the checked one otherwise
-        auto node = [&](auto fastTag, auto nodeTag, int c, int j, float cofP) -> float {
+        auto node = [&](auto fastTag, auto nodeTag, int c, int j, float hP) -> float {
             constexpr int NK = decltype(nodeTag)::value;
             const int lc = c + kEC, lr = j - js;
             if constexpr (decltype(fastTag)::value) {
-                return advect_node_fast<NK, NK, SW>(lds, plane, lc, lr, cofP, cofT);
+                return advect_node_fast<NK, NK, SW>(lds, plane, lc, lr, hP, hT);
             } else {
                 const Rsrc src = (NK == K_VPHI) ? up : ((NK == K_VTHETA) ? ut : rh);
-                return advect_node_st<NK, NK, SW>(st, up, ut, src, g, col(c), j, lc, lr, cofP, cofT);
+                return advect_node_st<NK, NK, SW>(st, up, ut, src, g, col(c), j, lc, lr, hP, hT);
             }
         };
         using KPhi = std::integral_constant<int, K_VPHI>;
@@ -451,9 +509,9 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
                 for (int idx = tid; idx < R * W; idx += nthr) {
                     const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
                     const int j = j0 + r, o = j * nP + c0 + c;
-                    oup[o] = node(F, KPhi{}, c, j, cofPC[j]);
-                    if (j < nT - 1) out[o] = node(F, KTheta{}, c, j, cofPT[j]);
-                    const float rr = node(F, KRho{}, c, j, cofPC[j]);
+                    oup[o] = node(F, KPhi{}, c, j, 0.5f * cofPC[j]);
+                    if (j < nT - 1) out[o] = node(F, KTheta{}, c, j, 0.5f * cofPT[j]);
+                    const float rr = node(F, KRho{}, c, j, 0.5f * cofPC[j]);
                     orh[o] = rr;
                     flag_bad(s.status, !isfinite(rr));
                 }
@@ -468,7 +526,8 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
 
         const int uLo = j0, uHi = min(j0 + R, nT - 1);
         const int vLo = max(j0 - 1, 0), vHi = min(j0 + R, nT - 2);
-        float* U = lds + 3 * plane;      // (R+1) x W2: u_phi*, columns c0-1 .. c0+W
+        // the rho nodes go first; their staged plane then holds U, V, A0, AL
+        float* U = lds + 2 * plane;      // (R+1) x W2: u_phi*, columns c0-1 .. c0+W
         float* V = U + (R + 1) * W2;     // (R+2) x W1: u_theta*, columns c0-1 .. c0+W-1
         float* A0 = V + (R + 2) * W1;    // W1: u_theta* of row 0 at the antipodal columns
         float* AL = A0 + W1;             // W1: u_theta* of row nT-2 at the antipodal columns
@@ -479,6 +538,13 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         // one backtrace per item: the loops end at wavefront granularity, with
         // no duplicated tail items (occupancy hides the LDS latency)
         auto advect_all = [&](auto F) {
+            for (int idx = tid; idx < nR; idx += nthr) {
+                const int r = (int)__umulhi((unsigned)idx, mW), c = idx - __mul24(r, W), j = j0 + r;
+                const float a = node(F, KRho{}, c, j, cofRow[0][r + 1]);
+                orh[j * nP + c0 + c] = a;
+                flag_bad(s.status, !isfinite(a));
+            }
+            __syncthreads();  // the staged rho plane is dead from here on
             for (int idx = tid; idx < nU; idx += nthr) {
                 const int r = (int)__umulhi((unsigned)idx, mW2), c = idx - __mul24(r, W2) - 1, j = uLo + r;
                 U[idx] = node(F, KPhi{}, c, j, cofRow[0][j - j0 + 1]);
@@ -486,12 +552,6 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
             for (int idx = tid; idx < nV; idx += nthr) {
                 const int r = (int)__umulhi((unsigned)idx, mW1), c = idx - __mul24(r, W1) - 1, j = vLo + r;
                 V[idx] = node(F, KTheta{}, c, j, cofRow[1][j - j0 + 1]);
-            }
-            for (int idx = tid; idx < nR; idx += nthr) {
-                const int r = (int)__umulhi((unsigned)idx, mW), c = idx - __mul24(r, W), j = j0 + r;
-                const float a = node(F, KRho{}, c, j, cofRow[0][r + 1]);
-                orh[j * nP + c0 + c] = a;
-                flag_bad(s.status, !isfinite(a));
             }
         };
         if (fast)
@@ -506,7 +566,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
                 int io = col(c - 1) + half;
                 io -= (io >= nP) ? nP : 0;
                 const int j = s0 ? 0 : nT - 2;
-                const float a = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, g, io, j, cofPT[j], cofT);
+                const float a = advect_node<K_VTHETA, K_VTHETA>(up, ut, ut, g, io, j, 0.5f * cofPT[j], hT);
                 (s0 ? A0 : AL)[c] = a;
             }
         }
@@ -535,16 +595,13 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
             VP[idx] = fmaf(G * x, x, vb);
         }
         __syncthreads();
-        // assignThetaKernel (rows j0..uHi-1, columns c0..c0+W-1) and
-        // assignPhiKernel (rows j0..j0+R-1)
-        const int nT2 = (uHi - j0) * W;
-        for (int idx = tid; idx < nT2; idx += nthr) {
-            const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
-            out[(j0 + r) * nP + c0 + c] = 0.5f * (VP[r * W1 + c + 1] + VP[(r + 1) * W1 + c + 1]);
-        }
+        // assignPhiKernel (rows j0..j0+R-1) and assignThetaKernel (rows
+        // j0..uHi-1), columns c0..c0+W-1
         for (int idx = tid; idx < nR; idx += nthr) {
             const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
-            oup[(j0 + r) * nP + c0 + c] = 0.5f * (X[r * W1 + c] + X[r * W1 + c + 1]);
+            const int o = (j0 + r) * nP + c0 + c, x = r * W1 + c;
+            oup[o] = 0.5f * (X[x] + X[x + 1]);
+            if (j0 + r < uHi) out[o] = 0.5f * (VP[x + 1] + VP[x + W1 + 1]);
         }
         __syncthreads();
     }

@@ -57,6 +57,8 @@ void release_tables(int device, int nTheta, double dt, double radius);
 // (7 * nTheta * nPhi floats: two state copies of three fields, then the
 // spectrum).
 void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& sd, float* mem);
+// Sets d.advCols = W and the division constants of its advection tiles.
+void set_tile_cols(SimDesc& d, int W);
 size_t sim_floats(int nTheta, int nPhi);
 
 // The launch sequence of one step over a set of simulations.

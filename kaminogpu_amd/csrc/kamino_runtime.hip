@@ -55,9 +55,12 @@ static int factor_m(int M, int* fac) {
 // advected u_phi and u_theta rows and the antipodal rows (the cubic roots
 // reuse the staged block)
 static size_t adv_lds(int R, int W, int SW, bool geometric) {
-    size_t n = (size_t)3 * (R + 2 * kAdvHaloRows) * SW;
-    if (geometric) n += (size_t)(R + 1) * (W + 2) + (R + 2) * (W + 1) + 2 * (W + 1);
-    return n * sizeof(float);
+    // three staged planes; with the geometric update the advected u_phi*,
+    // u_theta* and antipodal rows go over the third (rho) plane once the rho
+    // nodes are done, spilling past it when larger
+    const size_t plane = (size_t)(R + 2 * kAdvHaloRows) * SW;
+    const size_t uv = geometric ? (size_t)(R + 1) * (W + 2) + (R + 2) * (W + 1) + 2 * (W + 1) : 0;
+    return (2 * plane + std::max(plane, uv)) * sizeof(float);
 }
 // Lanes per row in the row FFT (a quarter of M, 8..64, power of two); a
 // 256-thread block holds 256 / group lane groups, each with a scratch row.
@@ -246,13 +249,16 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
     d.fftRows = p.fftRows;
     d.magicP = (unsigned)((((uint64_t)1 << 32) + (uint64_t)p.nP - 1) / (uint64_t)p.nP);
     d.invNP = 1.0f / (float)p.nP;
-    d.advCols = p.advCols;
+    set_tile_cols(d, p.advCols);
     d.radius = (float)sd.radius;
     d.dt = (float)sd.dt;
-    for (int k = 0; k < 3; ++k) {
-        const uint64_t w = (uint64_t)p.advCols + k;
-        d.magicW[k] = (unsigned)((((uint64_t)1 << 32) + w - 1) / w);
-    }
+}
+
+void set_tile_cols(SimDesc& d, int W) {
+    d.advCols = W;
+    const int SC = W + 2 * kAdvHaloCols;
+    const uint64_t div[5] = {(uint64_t)W, (uint64_t)W + 1, (uint64_t)W + 2, (uint64_t)SC, (uint64_t)SC / 4};
+    for (int k = 0; k < 5; ++k) d.magicW[k] = (unsigned)((((uint64_t)1 << 32) + div[k] - 1) / div[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -301,12 +307,12 @@ int DeviceBatch::reserve(size_t ns, size_t nt, hipStream_t st) {
 // simulation while a kernel would get fewer than two workgroups per CU.  The
 // height changes only how rows are grouped, never a value.
 static int shrink_until_busy(const std::vector<Plan>& plans, std::vector<int>& rows, int minTiles,
-                             bool advection) {
+                             const int* advCols) {
     for (;;) {
         long tiles = 0;
         bool canShrink = false;
         for (size_t s = 0; s < plans.size(); ++s) {
-            const long cols = advection ? (plans[s].nP + plans[s].advCols - 1) / plans[s].advCols : 1;
+            const long cols = advCols ? (plans[s].nP + advCols[s] - 1) / advCols[s] : 1;
             tiles += cols * ((plans[s].nT + rows[s] - 1) / rows[s]);
             canShrink |= rows[s] > 1;
         }
@@ -315,32 +321,47 @@ static int shrink_until_busy(const std::vector<Plan>& plans, std::vector<int>& r
     }
 }
 
-int DeviceBatch::build(const std::vector<SimDesc>& sims, const std::vector<Plan>& plans,
+int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Plan>& plans,
                        hipStream_t st) {
     hTiles_.clear();
     tri_.clear();
     advLds_ = fftLds_ = 0;
-    advStride_ = adv_stride(8);
-    for (const Plan& p : plans) advStride_ = std::max(advStride_, adv_stride(p.advCols));
     const int minTiles = 2 * device_cus();
-    std::vector<int> advRows(plans.size()), fftRows(plans.size());
-    for (size_t s = 0; s < plans.size(); ++s) {
+    const size_t n = plans.size();
+    std::vector<int> advRows(n), advCols(n), fftRows(n);
+    for (size_t s = 0; s < n; ++s) {
         advRows[s] = plans[s].advRows;
+        advCols[s] = plans[s].advCols;
         fftRows[s] = plans[s].fftRows;
     }
-    shrink_until_busy(plans, advRows, minTiles, true);
-    shrink_until_busy(plans, fftRows, minTiles, false);
+    // One advection launch stages every tile at the widest member's row
+    // stride and with the largest member's LDS.  When members were tuned to
+    // different widths, the wide ones (alone a launch too small to fill the
+    // GPU) are restaged as narrow tiles of the same area, so that the narrow
+    // majority keeps its occupancy.  Tile shapes never change a value.
+    std::vector<SimDesc> sims(simsIn);
+    bool narrow = false, wide = false;
+    for (size_t s = 0; s < n; ++s) (advCols[s] > kAdvMaxCols / 2 ? wide : narrow) = true;
+    if (narrow && wide)
+        for (size_t s = 0; s < n; ++s)
+            if (advCols[s] > kAdvMaxCols / 2) {
+                advRows[s] = std::min(2 * advRows[s], kAdvMaxRows);
+                advCols[s] = kAdvMaxCols / 2;
+                set_tile_cols(sims[s], advCols[s]);
+            }
+    advStride_ = adv_stride(wide && !narrow ? kAdvMaxCols : kAdvMaxCols / 2);
+    shrink_until_busy(plans, advRows, minTiles, advCols.data());
+    shrink_until_busy(plans, fftRows, minTiles, nullptr);
     // advection tiles, in batch order (tallest simulations first); the last
     // column tile of a row band overlaps its neighbour instead of running short
     advOff_ = 0;
-    for (size_t s = 0; s < sims.size(); ++s) {
+    for (size_t s = 0; s < n; ++s) {
         const Plan& p = plans[s];
+        const int W = advCols[s];
         for (int j = 0; j < p.nT; j += advRows[s])
-            for (int c = 0; c < p.nP; c += p.advCols)
-                hTiles_.push_back(Tile{(int)s, j, std::min(advRows[s], p.nT - j),
-                                       std::min(c, p.nP - p.advCols)});
-        advLds_ = std::max(advLds_, adv_lds(advRows[s], p.advCols, advStride_,
-                                            (sims[s].flags & SIM_GEOMETRIC) != 0));
+            for (int c = 0; c < p.nP; c += W)
+                hTiles_.push_back(Tile{(int)s, j, std::min(advRows[s], p.nT - j), std::min(c, p.nP - W)});
+        advLds_ = std::max(advLds_, adv_lds(advRows[s], W, advStride_, (sims[s].flags & SIM_GEOMETRIC) != 0));
     }
     nAdv_ = (int)hTiles_.size();
     fftOff_ = (int)hTiles_.size();

@@ -41,6 +41,27 @@ def test_uneven_batch_equals_alone_bitwise():
     log(f"[engine] uneven batch of {len(reqs)} sims bitwise equal to alone")
 
 
+def test_mixed_tile_widths_equal_alone_bitwise():
+    # 256 tuned to 128-column tiles, the others to 64: alone, 256 stages at the
+    # wide stride; in the batch it is restaged as 64-column tiles
+    sizes = [256, 128, 64]
+    reqs = [make_request(nt, 300 + k) for k, nt in enumerate(sizes)]
+    try:
+        K.tune_set(256, 8, 128, 4)
+        K.tune_set(128, 16, 64, 4)
+        with K.Engine() as eng:
+            ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=4) for r in reqs]
+            eng.step(4)
+            got = [eng.fetch(i, r["nt"]) for r, i in zip(reqs, ids)]
+        for r, g in zip(reqs, got):
+            for a, b in zip(g, alone(r["nt"], r["dt"], r["state"], 4)):
+                np.testing.assert_array_equal(a, b)
+    finally:
+        K.tune_set(256, 0, 0, 0)
+        K.tune_set(128, 0, 0, 0)
+    log("[engine] batch mixing 128- and 64-column tiles bitwise equal to alone")
+
+
 def test_graph_and_direct_launches_agree():
     reqs = [make_request(nt, 7 + k) for k, nt in enumerate([64, 96, 64])]
     out = []
