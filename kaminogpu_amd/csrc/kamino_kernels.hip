@@ -168,6 +168,9 @@ __device__ __forceinline__ float advect_node(Rsrc up, Rsrc ut, Rsrc src, const G
 // the reference's own branch choice is evaluated in FP64.
 // ---------------------------------------------------------------------------
 __device__ double ref_root3_pos(double x) {
+    // x > 0.  A non-finite x (from an overflowed velocity) is returned as is,
+    // for the watchdog to flag; scaling +inf into [1, 8] would never end.
+    if (!(x <= 1.7976931348623157e308)) return x;
     double s = 1.;
     while (x < 1.) {
         x *= 8.;

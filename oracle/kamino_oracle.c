@@ -13,6 +13,7 @@
  */
 #include "kamino_oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -274,6 +275,8 @@ void ko_advect(int nTheta, int nPhi, double radius, double dt,
 /* Cubic -- KaminoCore.cu:386-454                                            */
 /* ------------------------------------------------------------------------ */
 static double ref_root3_pos(double x) { /* _root3, KaminoCore.cu:386-407 */
+    /* x > 0; +inf / NaN return as is (the reference's scaling loop never ends on +inf) */
+    if (!(x <= DBL_MAX)) return x;
     double s = 1.;
     while (x < 1.) {
         x *= 8.;

@@ -192,6 +192,13 @@ def test_cubic_special_cases():
     assert x == pytest.approx(0.7 / (1 + G * 0.2), rel=1e-6)
 
 
+def test_cubic_overflowed_input_returns_non_finite():
+    # an infinite u_phi in the one-real-root Cardano branch: the cube root's
+    # scaling loop would never end on +inf; the kernel carries the same guard
+    for c in (np.inf, -np.inf):
+        assert not np.isfinite(O.solve_cubic(0.0, -4.0, c))
+
+
 # ------------------------------------------------------------ geometric ----
 def test_geometric_closed_form_at_dt_zero():
     nt, nph = 8, 16
