@@ -57,6 +57,7 @@ struct SimDesc {
     int advRows;           // rows per advection tile
     int fftRows;           // rows per FFT tile
     unsigned magicP;       // ceil(2^32 / nPhi): i / nPhi == umulhi(i, magicP) for i < 2^16
+    unsigned magicP2;      // ceil(2^32 / (nPhi / 2))
     float invNP;           // 1 / nPhi
     int advCols;           // columns per advection tile (W <= min(nPhi, 255))
     unsigned magicW[5];    // ceil(2^32 / d) for d = W, W + 1, W + 2, the staged width W + 8 and its quarter

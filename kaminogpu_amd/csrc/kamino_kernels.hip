@@ -37,6 +37,9 @@ __device__ __forceinline__ Rsrc make_rsrc(const float* p, int nfloats) {
 __device__ __forceinline__ float ldf(Rsrc r, int elem) {
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, elem * 4, 0, 0));
 }
+__device__ __forceinline__ float2 ldf2(Rsrc r, int elem) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 4, 0, 0));
+}
 __device__ __forceinline__ float4 ldf4(Rsrc r, int elem) {
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 4, 0, 0));
 }
@@ -806,6 +809,28 @@ __device__ void wave_fft(float2* row, float2* scratch, const float2* tw, const S
     }
 }
 
+// Memory phases of the FFT kernels issue a batch of loads per thread before the
+// first use, so that a tile's HBM latency is paid once per batch rather than
+// once per item.
+constexpr int kDivBatch = 4;   // row points per lane in k_div_fft
+constexpr int kSpecBatch = 8;  // spectrum entries per thread in k_ifft_grad
+constexpr int kGradBatch = 4;  // column pairs per thread in k_ifft_grad
+
+// Stages the nPhi twiddles of s into LDS, two per float4 (nPhi is even).
+__device__ __forceinline__ void stage_twiddles(float2* twl, const SimDesc& s, int tid, int nthr) {
+    const int n4 = s.nPhi >> 1;
+    const Rsrc tr = make_rsrc(reinterpret_cast<const float*>(s.twN), 2 * s.nPhi);
+    float4* dst = reinterpret_cast<float4*>(twl);
+    for (int i0 = tid; i0 < n4; i0 += 4 * nthr) {
+        float4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = ldf4(tr, 4 * (i0 + q * nthr));
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (i0 + q * nthr < n4) dst[i0 + q * nthr] = v[q];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_div_fft: divergence (fillDivergenceKernel, KaminoCore.cu:587-638) of R
 // rows, packed two reals per complex (z_k = d_2k + i d_2k+1), FFT of length
@@ -817,7 +842,7 @@ __device__ void wave_fft(float2* row, float2* scratch, const float2* tw, const S
 __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sims,
                                                  const Tile* __restrict__ tiles, int ntiles,
                                                  int parity) {
-    extern __shared__ float2 zlds[];
+    extern __shared__ __attribute__((aligned(16))) float2 zlds[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
@@ -833,32 +858,38 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
         float2* twl = zlds;                                        // nPhi twiddles
         float2* zbuf = zlds + nP;                                  // R rows of M
         float2* scratch = zbuf + (size_t)(R + grp) * M;            // this lane group's row
-        const float2* __restrict__ twg = reinterpret_cast<const float2*>(s.twN);
-        for (int i = tid; i < nP; i += nthr) twl[i] = twg[i];
+        stage_twiddles(twl, s, tid, nthr);
+        const Rsrc upR = make_rsrc(up, nT * nP), utR = make_rsrc(ut, (nT - 1) * nP);
         __syncthreads();
         const int nPass = (R + ngrp - 1) / ngrp;                   // uniform trip count
         for (int it = 0; it < nPass; ++it) {
             const int r = it * ngrp + grp;
             const bool active = r < R;
             const int j = j0 + (active ? r : 0);
-            const float* urow = up + (size_t)j * nP;
             const float fs = divS[j], fn = divN[j], fi = invHs[j];
             float2* row = zbuf + (size_t)(active ? r : 0) * M;
-            for (int k = lane; active && k < M; k += gs) {
-                const int i0 = 2 * k, i1 = i0 + 1, i2 = (i1 + 1 == nP) ? 0 : i1 + 1;
-                const float uw0 = urow[i0], ue0 = urow[i1], ue1 = urow[i2];
-                float vn0 = 0.f, vn1 = 0.f, vs0 = 0.f, vs1 = 0.f;
-                if (j > 0) {
-                    vn0 = ut[(size_t)(j - 1) * nP + i0];
-                    vn1 = ut[(size_t)(j - 1) * nP + i1];
+            // kDivBatch points per lane at a time, all their loads issued
+            // before the first use; rows beyond the field (j - 1 < 0, j = nT - 1
+            // for u_theta) read as 0 through the buffer range check
+            const int on = (j - 1) * nP, oc = j * nP;
+            for (int kb = 0; kb < M; kb += kDivBatch * gs) {
+                float2 uw[kDivBatch], vn[kDivBatch], vs[kDivBatch];
+                float ue1[kDivBatch];
+#pragma unroll
+                for (int q = 0; q < kDivBatch; ++q) {
+                    const int i0 = 2 * (kb + q * gs + lane), i2 = (i0 + 2 == nP) ? 0 : i0 + 2;
+                    uw[q] = ldf2(upR, oc + i0);
+                    ue1[q] = ldf(upR, oc + i2);
+                    vn[q] = ldf2(utR, j > 0 ? on + i0 : -1);
+                    vs[q] = ldf2(utR, oc + i0);
                 }
-                if (j < nT - 1) {
-                    vs0 = ut[(size_t)j * nP + i0];
-                    vs1 = ut[(size_t)j * nP + i1];
+#pragma unroll
+                for (int q = 0; q < kDivBatch; ++q) {
+                    const int k = kb + q * gs + lane;
+                    const float d0 = (fs * vs[q].x - fn * vn[q].x) + fi * (uw[q].y - uw[q].x);
+                    const float d1 = (fs * vs[q].y - fn * vn[q].y) + fi * (ue1[q] - uw[q].y);
+                    if (active && k < M) row[k] = make_float2(d0, d1);
                 }
-                const float d0 = (fs * vs0 - fn * vn0) + fi * (ue0 - uw0);
-                const float d1 = (fs * vs1 - fn * vn1) + fi * (ue1 - ue0);
-                row[k] = make_float2(d0, d1);
             }
             wave_sync();
             wave_fft<-1>(row, scratch, twl, s, lane, gs, active);
@@ -1194,7 +1225,7 @@ __global__ __launch_bounds__(256) void k_tridiag_mixed(const SimDesc* __restrict
 __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ sims,
                                                    const Tile* __restrict__ tiles, int ntiles,
                                                    int parity) {
-    extern __shared__ float2 zlds[];
+    extern __shared__ __attribute__((aligned(16))) float2 zlds[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
@@ -1206,27 +1237,39 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
         float2* tw = zlds;                                          // nPhi twiddles
         float2* zbuf = zlds + nP;                                   // Rp rows of M
         float2* scratch = zbuf + (size_t)(Rp + grp) * M;            // this lane group's row
-        const float2* __restrict__ spec = reinterpret_cast<const float2*>(s.spec);
-        const float2* __restrict__ twg = reinterpret_cast<const float2*>(s.twN);
-        for (int i = tid; i < nP; i += nthr) tw[i] = twg[i];
+        const Rsrc specR = make_rsrc(s.spec, 2 * M * nT);
+        stage_twiddles(tw, s, tid, nthr);
         __syncthreads();
-        const unsigned magicR = (unsigned)((((uint64_t)1 << 32) + Rp - 1) / Rp);
-        for (int idx = tid; idx < Rp * M; idx += nthr) {
-            const int n = (Rp == 1) ? idx : (int)__umulhi((unsigned)idx, magicR), r = idx - n * Rp, j = j0 + r;
-            const float2 Pn = spec[(size_t)n * nT + j];
-            float2 Y;
-            if (n == 0) {
-                Y = make_float2(Pn.y, -Pn.y);  // P_0 = 0 (mode dropped), P_M = Nyquist
-            } else {
-                const float2 Pm = spec[(size_t)(M - n) * nT + j];
-                const float2 ev = make_float2(Pn.x + Pm.x, Pn.y - Pm.y);
-                const float2 od = make_float2(Pn.x - Pm.x, Pn.y + Pm.y);
-                float2 w = tw[n];
-                w.y = -w.y;  // exp(+i pi n / M)
-                const float2 o = cmul(w, od);
-                Y = make_float2(ev.x - o.y, ev.y + o.x);
+        // ceil(2^32 / Rp) (Rp = 1 splits nothing); idx / Rp is exact for idx < 2^16
+        const unsigned magicR = (Rp == 1) ? 0u : 0xffffffffu / (unsigned)Rp + 1u;
+        for (int idx0 = tid; idx0 < Rp * M; idx0 += kSpecBatch * nthr) {
+            float2 Pn[kSpecBatch], Pm[kSpecBatch];
+#pragma unroll
+            for (int q = 0; q < kSpecBatch; ++q) {
+                const int idx = idx0 + q * nthr;
+                const int n = (Rp == 1) ? idx : (int)__umulhi((unsigned)idx, magicR), j = j0 + idx - n * Rp;
+                // past the spectrum (idx beyond the tile, or M - n = M) reads as 0
+                Pn[q] = ldf2(specR, 2 * (n * nT + j));
+                Pm[q] = ldf2(specR, 2 * ((M - n) * nT + j));
             }
-            zbuf[(size_t)r * M + n] = Y;
+#pragma unroll
+            for (int q = 0; q < kSpecBatch; ++q) {
+                const int idx = idx0 + q * nthr;
+                if (idx >= Rp * M) break;
+                const int n = (Rp == 1) ? idx : (int)__umulhi((unsigned)idx, magicR), r = idx - n * Rp;
+                float2 Y;
+                if (n == 0) {
+                    Y = make_float2(Pn[q].y, -Pn[q].y);  // P_0 = 0 (mode dropped), P_M = Nyquist
+                } else {
+                    const float2 ev = make_float2(Pn[q].x + Pm[q].x, Pn[q].y - Pm[q].y);
+                    const float2 od = make_float2(Pn[q].x - Pm[q].x, Pn[q].y + Pm[q].y);
+                    float2 w = tw[n];
+                    w.y = -w.y;  // exp(+i pi n / M)
+                    const float2 o = cmul(w, od);
+                    Y = make_float2(ev.x - o.y, ev.y + o.x);
+                }
+                zbuf[(size_t)r * M + n] = Y;
+            }
         }
         __syncthreads();
         const int nPass = (Rp + ngrp - 1) / ngrp;
@@ -1238,22 +1281,45 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
         const float* p = reinterpret_cast<const float*>(zbuf);  // p[r][i], row stride nP
         float* __restrict__ up = s.state[parity][F_UPHI];
         float* __restrict__ ut = s.state[parity][F_UTHETA];
-        const float* __restrict__ invHs = s.rowf + RT_INVHS * nT;
+        const Rsrc upR = make_rsrc(up, nT * nP), utR = make_rsrc(ut, (nT - 1) * nP);
+        const Rsrc hsR = make_rsrc(s.rowf + RT_INVHS * nT, nT);
         const float invH = s.invH;
-        for (int idx = tid; idx < R * nP; idx += nthr) {
-            const int r = (int)__umulhi((unsigned)idx, s.magicP), i = idx - r * nP, j = j0 + r;
-            const int il = (i == 0) ? nP - 1 : i - 1;
-            const float pc = p[(size_t)r * nP + i];
-            const size_t o = (size_t)j * nP + i;
-            const float nu = up[o] - (pc - p[(size_t)r * nP + il]) * invHs[j];
-            up[o] = nu;
-            bool bad = !isfinite(nu);
-            if (j < nT - 1) {
-                const float nv = ut[o] - (p[(size_t)(r + 1) * nP + i] - pc) * invH;
-                ut[o] = nv;
-                bad |= !isfinite(nv);
+        // column pairs (i, i + 1) (nPhi is even); a batch's u loads all go out
+        // before its first store, and u_theta of the last row reads as 0
+        const int half = nP >> 1, nPair = R * half;
+        const unsigned magicH = s.magicP2;  // idx / (nPhi / 2)
+        for (int idx0 = tid; idx0 < nPair; idx0 += kGradBatch * nthr) {
+            float2 u[kGradBatch], v[kGradBatch];
+            float hs[kGradBatch];
+#pragma unroll
+            for (int q = 0; q < kGradBatch; ++q) {
+                const int idx = idx0 + q * nthr;
+                const int r = (int)__umulhi((unsigned)idx, magicH), i = 2 * (idx - r * half);
+                const int o = (j0 + r) * nP + i;
+                u[q] = ldf2(upR, o);
+                v[q] = ldf2(utR, o);
+                hs[q] = ldf(hsR, j0 + r);
             }
-            flag_bad(s.status, bad);
+#pragma unroll
+            for (int q = 0; q < kGradBatch; ++q) {
+                const int idx = idx0 + q * nthr;
+                if (idx >= nPair) break;
+                const int r = (int)__umulhi((unsigned)idx, magicH), i = 2 * (idx - r * half), j = j0 + r;
+                const int il = (i == 0) ? nP - 1 : i - 1;
+                const float* pr = p + (size_t)r * nP;
+                const float p0 = pr[i], p1 = pr[i + 1];
+                const float2 nu = make_float2(u[q].x - (p0 - pr[il]) * hs[q], u[q].y - (p1 - p0) * hs[q]);
+                const size_t o = (size_t)j * nP + i;
+                *reinterpret_cast<float2*>(up + o) = nu;
+                bool bad = !isfinite(nu.x) || !isfinite(nu.y);
+                if (j < nT - 1) {
+                    const float2 nv = make_float2(v[q].x - (pr[nP + i] - p0) * invH,
+                                                  v[q].y - (pr[nP + i + 1] - p1) * invH);
+                    *reinterpret_cast<float2*>(ut + o) = nv;
+                    bad |= !isfinite(nv.x) || !isfinite(nv.y);
+                }
+                flag_bad(s.status, bad);
+            }
         }
         __syncthreads();
     }

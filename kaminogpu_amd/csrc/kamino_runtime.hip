@@ -248,6 +248,7 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
     d.advRows = p.advRows;
     d.fftRows = p.fftRows;
     d.magicP = (unsigned)((((uint64_t)1 << 32) + (uint64_t)p.nP - 1) / (uint64_t)p.nP);
+    d.magicP2 = (unsigned)((((uint64_t)1 << 32) + (uint64_t)p.M - 1) / (uint64_t)p.M);
     d.invNP = 1.0f / (float)p.nP;
     set_tile_cols(d, p.advCols);
     d.radius = (float)sd.radius;

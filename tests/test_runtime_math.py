@@ -20,3 +20,10 @@ def test_flat_loops_fit_the_magic_range():
     assert (64 + 6) * (128 + 8) < (1 << 16)
     assert (64 + 1) * (128 + 2) < (1 << 16)
     assert max((256 // L) * nt for L, nt in [(8, 64), (16, 128), (32, 256), (64, 1024)]) < (1 << 16)
+
+
+def test_32_bit_form_of_the_magic_constant():
+    # k_ifft_grad forms ceil(2^32 / d) on the device as 0xffffffff / d + 1
+    # (d = rows + 1 >= 2): equal for every divisor, powers of two included
+    for d in range(2, 2049):
+        assert 0xFFFFFFFF // d + 1 == ((1 << 32) + d - 1) // d, d
