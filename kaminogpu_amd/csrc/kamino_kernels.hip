@@ -40,6 +40,9 @@ __device__ __forceinline__ float ldf(Rsrc r, int elem) {
 __device__ __forceinline__ float2 ldf2(Rsrc r, int elem) {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 4, 0, 0));
 }
+__device__ __forceinline__ double ldd(Rsrc r, int elem) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8, 0, 0));
+}
 __device__ __forceinline__ float4 ldf4(Rsrc r, int elem) {
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 4, 0, 0));
 }
@@ -953,6 +956,8 @@ __device__ __forceinline__ double shfl_grp(double v, int srcLaneInGroup, int lan
 // at [k][l] (a padded row per k keeps the staging writes spread over the
 // banks).  RESID additionally writes, per mode, ||T x - d|| / ||d|| of the
 // FP64 solution x before it is rounded to FP32 (to SimDesc::resid).
+constexpr int kTriBatch = 4;  // float4 spectrum loads per thread in flight while staging
+
 template <int L, int KMAX, bool RESID>
 __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const Tile& tile,
                                          float2* zsp, double* csp, double* ssp) {
@@ -972,26 +977,59 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
         const bool active = g < tile.rows;
         const int n = tile.row0 + (active ? g : 0);
         const int K = (nT + L - 1) / L;
-        // i / K and e / nT as multiply-high (exact for dividends below 2^16;
-        // K = 1, whose multiplier 2^32 does not fit, is taken apart)
-        const unsigned magicK = (unsigned)((((uint64_t)1 << 32) + K - 1) / K);
-        const unsigned magicT = (unsigned)((((uint64_t)1 << 32) + nT - 1) / nT);
+        // i / K and e / nT as multiply-high (exact for dividends below 2^16):
+        // ceil(2^32 / K) in its 32-bit form (K = 1, whose multiplier 2^32 does
+        // not fit, is taken apart), and e / nT = 2e / nPhi
+        const unsigned magicK = (K == 1) ? 0u : 0xffffffffu / (unsigned)K + 1u;
+        auto split = [&](int i, int& li, int& ki) {
+            li = (K == 1) ? i : (int)__umulhi((unsigned)i, magicK);
+            ki = i - li * K;
+        };
         const int r0 = l * K;
         int cnt = nT - r0;
         cnt = cnt < 0 ? 0 : (cnt > K ? K : cnt);
         if (!active) cnt = 0;
-        float2* __restrict__ blk = reinterpret_cast<float2*>(s.spec) + (size_t)tile.row0 * nT;
-        const double* __restrict__ cot2h = s.rowd + RD_COT2H * nT;
-        const double* __restrict__ isin2 = s.rowd + RD_INVSIN2 * nT;
-        for (int i = tid; i < nT; i += 256) {
-            const int li = (K == 1) ? i : (int)__umulhi((unsigned)i, magicK), ki = i - li * K;
-            cs(ki, li) = cot2h[i];
-            ss(ki, li) = isin2[i];
+        // staging: a thread's loads of a batch all go out before its first
+        // LDS write; reads past the tables or the spectrum return 0 and are
+        // not kept
+        const Rsrc rowR = make_rsrc(reinterpret_cast<const float*>(s.rowd), 2 * RD_COUNT * nT);
+        for (int i0 = tid; i0 < nT; i0 += 4 * 256) {
+            double c[4], v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                c[q] = ldd(rowR, RD_COT2H * nT + i0 + q * 256);
+                v[q] = ldd(rowR, RD_INVSIN2 * nT + i0 + q * 256);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int i = i0 + q * 256;
+                if (i >= nT) break;
+                int li, ki;
+                split(i, li, ki);
+                cs(ki, li) = c[q];
+                ss(ki, li) = v[q];
+            }
         }
-        for (int e = tid; e < tile.rows * nT; e += 256) {
-            const int ge = (int)__umulhi((unsigned)e, magicT), i = e - ge * nT;
-            const int li = (K == 1) ? i : (int)__umulhi((unsigned)i, magicK), ki = i - li * K;
-            zs(ge, ki, li) = blk[e];
+        // the tile's modes are consecutive spectrum rows: pairs of entries
+        // as float4 (row0 * nT is even, row0 being a multiple of 256 / L)
+        const Rsrc specR = make_rsrc(s.spec, 2 * s.M * nT);
+        const int total = tile.rows * nT, b2 = 2 * tile.row0 * nT;
+        for (int e0 = 2 * tid; e0 < total; e0 += 2 * 256 * kTriBatch) {
+            float4 v[kTriBatch];
+#pragma unroll
+            for (int q = 0; q < kTriBatch; ++q) v[q] = ldf4(specR, b2 + 2 * (e0 + 512 * q));
+#pragma unroll
+            for (int q = 0; q < kTriBatch; ++q)
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int e = e0 + 512 * q + h;
+                    if (e < total) {
+                        const int ge = (int)__umulhi((unsigned)(2 * e), s.magicP), i = e - ge * nT;
+                        int li, ki;
+                        split(i, li, ki);
+                        zs(ge, ki, li) = h ? make_float2(v[q].z, v[q].w) : make_float2(v[q].x, v[q].y);
+                    }
+                }
         }
         __syncthreads();
         const double h2 = s.invH2;
@@ -1176,10 +1214,22 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                 if (k < cnt) zs(g, k, l) = make_float2((float)xsr[k], (float)xsi[k]);
         }
         __syncthreads();
-        for (int e = tid; e < tile.rows * nT; e += 256) {
-            const int ge = (int)__umulhi((unsigned)e, magicT), i = e - ge * nT;
-            const int li = (K == 1) ? i : (int)__umulhi((unsigned)i, magicK), ki = i - li * K;
-            blk[e] = zs(ge, ki, li);
+        // write-back, pairs of entries as float4 where both lie in the tile
+        float* __restrict__ blk = s.spec + b2;
+        auto staged = [&](int e) -> float2 {
+            const int ge = (int)__umulhi((unsigned)(2 * e), s.magicP), i = e - ge * nT;
+            int li, ki;
+            split(i, li, ki);
+            return zs(ge, ki, li);
+        };
+        for (int e = 2 * tid; e < total; e += 512) {
+            const float2 x = staged(e);
+            if (e + 1 < total) {
+                const float2 y = staged(e + 1);
+                *reinterpret_cast<float4*>(blk + 2 * e) = make_float4(x.x, x.y, y.x, y.y);
+            } else {
+                *reinterpret_cast<float2*>(blk + 2 * e) = x;
+            }
         }
     }
 }
