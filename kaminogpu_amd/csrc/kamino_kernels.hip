@@ -407,7 +407,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         const float* __restrict__ cofPT = s.rowf + RT_COFPHI_T * nT;
         const float hT = 0.5f * s.cofTheta;  // half of theta cells per unit u_theta
         const int j0 = tile.row0, R = tile.rows, c0 = tile.col0;
-        const int W = s.advCols, W1 = W + 1, W2 = W + 2;
+        const int W = s.advCols;
         const unsigned mW = s.magicW[0], mW1 = s.magicW[1], mW2 = s.magicW[2];
         // tile column c (from -1) -> grid column, wrapping at the phi seam
         auto col = [&](int c) {
@@ -430,8 +430,15 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         // max |u_phi|, |u_theta| of the staged block and max half phi
         // coefficient of the tile's node rows (float bits)
         __shared__ unsigned umax[3];
-        // half phi coefficients of node rows j0-1 .. j0+R (u_phi / rho nodes, u_theta nodes)
-        __shared__ float cofRow[2][kAdvMaxRows + 2];
+        // half phi coefficients of node rows j0-1 .. j0+R (u_phi / rho nodes,
+        // u_theta nodes) and the geometric coefficient G of those rows, after
+        // the staged planes and the advected u block
+        const int W1 = W + 1, W2 = W + 2;
+        const int uvN = (s.flags & SIM_GEOMETRIC) ? (R + 1) * W2 + (R + 2) * W1 + 2 * W1 : 0;
+        float* cofRow[3];
+        cofRow[0] = lds + 2 * plane + max(plane, uvN);
+        cofRow[1] = cofRow[0] + (R + 2);
+        cofRow[2] = cofRow[1] + (R + 2);
         if (tid < 3) umax[tid] = 0u;
         __syncthreads();
         if (tid < R + 2) {  // node rows j0-1 .. j0+R, one per thread
@@ -440,6 +447,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
             const float cc = in ? 0.5f * cofPC[j] : 0.f, ct = in ? 0.5f * cofPT[j] : 0.f;
             cofRow[0][tid] = cc;
             cofRow[1][tid] = ct;
+            cofRow[2][tid] = in ? s.rowf[RT_G * nT + j] : 0.f;
             const float cm = fmaxf(cc, ct);
             atomicMax(&umax[2], __float_as_uint(cm != cm ? INFINITY : cm));
         }
@@ -452,21 +460,37 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
             // four columns per item: c0 - kEC, SC, nP and SW are multiples of
             // 4, so a float4 never straddles the phi seam and lands 16-byte
             // aligned in LDS
-            const int SC4 = SC >> 2;
+            // (two items per thread in flight: their loads all go out before
+            // the first LDS write)
+            const int SC4 = SC >> 2, n4 = SR * SC4;
             const unsigned mSC4 = s.magicW[4];  // e / SC4, e < 2^16
-            for (int e = tid; e < SR * SC4; e += nthr) {
-                const int r = (int)__umulhi((unsigned)e, mSC4), c = (e - __mul24(r, SC4)) << 2;
-                int gi = c0 - kEC + c;
-                gi += (gi < 0) ? nP : 0;
-                gi -= (gi >= nP) ? nP : 0;
-                // rows outside the field read as 0 (buffer range check) and are never sampled
-                const int o = (js + r) * nP + gi, l = r * SW + c;
-                const float4 a = ldf4(up, o), b = ldf4(ut, o);
-                *reinterpret_cast<float4*>(lds + l) = a;
-                *reinterpret_cast<float4*>(lds + plane + l) = b;
-                *reinterpret_cast<float4*>(lds + 2 * plane + l) = ldf4(rh, o);
-                mp = max(mp, max(max(absbits(a.x), absbits(a.y)), max(absbits(a.z), absbits(a.w))));
-                mt = max(mt, max(max(absbits(b.x), absbits(b.y)), max(absbits(b.z), absbits(b.w))));
+            for (int e0 = tid; e0 < n4; e0 += 2 * nthr) {
+                float4 a[2], b[2], d[2];
+                int l[2];
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int e = e0 + q * nthr;
+                    const int r = (int)__umulhi((unsigned)e, mSC4), c = (e - __mul24(r, SC4)) << 2;
+                    int gi = c0 - kEC + c;
+                    gi += (gi < 0) ? nP : 0;
+                    gi -= (gi >= nP) ? nP : 0;
+                    // rows outside the field (and items past the block) read as 0
+                    // through the buffer range check; the former are never sampled
+                    const int o = (js + r) * nP + gi;
+                    l[q] = r * SW + c;
+                    a[q] = ldf4(up, o);
+                    b[q] = ldf4(ut, o);
+                    d[q] = ldf4(rh, o);
+                }
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    if (e0 + q * nthr >= n4) break;
+                    *reinterpret_cast<float4*>(lds + l[q]) = a[q];
+                    *reinterpret_cast<float4*>(lds + plane + l[q]) = b[q];
+                    *reinterpret_cast<float4*>(lds + 2 * plane + l[q]) = d[q];
+                    mp = max(mp, max(max(absbits(a[q].x), absbits(a[q].y)), max(absbits(a[q].z), absbits(a[q].w))));
+                    mt = max(mt, max(max(absbits(b[q].x), absbits(b[q].y)), max(absbits(b[q].z), absbits(b[q].w))));
+                }
             }
         } else {
             const unsigned mSC = s.magicW[3];  // e / SC, e < 2^16
@@ -584,7 +608,6 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         // geometricFillKernel, one cell per item: the root x and v' = vbar + G x^2
         // of rows j0..uHi (the row below the tile only feeds the u_theta
         // restagger), both into the dead staged block
-        const float* __restrict__ Gt = s.rowf + RT_G * nT;
         const int nG = (uHi - j0 + 1) * W1;
         float* VP = X + (R + 1) * W1;  // (R+1) x W1: v'
         for (int idx = tid; idx < nG; idx += nthr) {
@@ -598,7 +621,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
                 vb = 0.75f * V[(nT - 2 - vLo) * W1 + c] + 0.25f * AL[c];
             else
                 vb = 0.5f * (V[(j - 1 - vLo) * W1 + c] + V[(j - vLo) * W1 + c]);
-            const float G = Gt[j];
+            const float G = cofRow[2][r + 1];
             const float x = geo_root(G, vb, ub);
             X[idx] = x;
             VP[idx] = fmaf(G * x, x, vb);
@@ -937,8 +960,9 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
 // reference and its result is discarded (cacheZeroComponents :672-680 and
 // shiftUKernel :682-703 subtract it); its slot carries the Nyquist mode.
 // ---------------------------------------------------------------------------
-// 1/x to FP64 precision: the hardware estimate refined by two Newton steps
-// (each squares the relative error), shorter than an IEEE division sequence.
+// 1/x to FP64 precision: the hardware estimate refined by one Newton step
+// (which squares its relative error), shorter than an IEEE division sequence;
+// the FP64 residual of the solve measures ~1e-16.
 __device__ __forceinline__ double rcp64(double x) {
     const double r = __builtin_amdgcn_rcp(x);
     return fma(r, fma(-x, r, 1.0), r);  // one Newton step doubles the hardware estimate's bits
