@@ -220,9 +220,12 @@ __device__ __forceinline__ float geo_root(float G, float vb, float ub) {
         const float a = G * G;
         float x = ub * __builtin_amdgcn_rcpf(b);
         if (a * x * x > b) x = copysignf(cbrtf(fabsf(ub) * __builtin_amdgcn_rcpf(a)), ub);
-        // at most five steps.  A lane stops updating once its step falls below
-        // FP32 resolution (so its result never depends on its neighbours), and
-        // the wavefront leaves when every lane has (most cells need one or two)
+        // at most five steps.  Converging from above, the error after a step
+        // of dx is at most 3 (dx / x)^2 |x| (f'' / 2f' <= 3 a |x| / (3 a x^2 + b)
+        // between the iterate and the root), so a lane stops once |dx| <= 1e-4 |x|
+        // (error below 3e-8 |x|, under FP32 resolution) and then no longer
+        // updates (its result never depends on its neighbours); the wavefront
+        // leaves when every lane has (most cells need one step)
         bool done = false;
         for (int k = 0; k < 5; ++k) {
             const float x2 = x * x;
@@ -231,7 +234,7 @@ __device__ __forceinline__ float geo_root(float G, float vb, float ub) {
             const float dx = f * __builtin_amdgcn_rcpf(fp);
             if (!done) {
                 x -= dx;
-                done = !(fabsf(dx) > 1e-7f * fabsf(x));
+                done = !(fabsf(dx) > 1e-4f * fabsf(x));
             }
             if (__all(done)) break;
         }
@@ -884,9 +887,7 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
         float2* twl = zlds;                                        // nPhi twiddles
         float2* zbuf = zlds + nP;                                  // R rows of M
         float2* scratch = zbuf + (size_t)(R + grp) * M;            // this lane group's row
-        stage_twiddles(twl, s, tid, nthr);
         const Rsrc upR = make_rsrc(up, nT * nP), utR = make_rsrc(ut, (nT - 1) * nP);
-        __syncthreads();
         const int nPass = (R + ngrp - 1) / ngrp;                   // uniform trip count
         for (int it = 0; it < nPass; ++it) {
             const int r = it * ngrp + grp;
@@ -909,6 +910,8 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
                     vn[q] = ldf2(utR, j > 0 ? on + i0 : -1);
                     vs[q] = ldf2(utR, oc + i0);
                 }
+                // the twiddles are staged while the first rows are in flight
+                if (it == 0 && kb == 0) stage_twiddles(twl, s, tid, nthr);
 #pragma unroll
                 for (int q = 0; q < kDivBatch; ++q) {
                     const int k = kb + q * gs + lane;
@@ -917,6 +920,7 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
                     if (active && k < M) row[k] = make_float2(d0, d1);
                 }
             }
+            if (it == 0) __syncthreads();  // twiddles visible to every lane group
             wave_sync();
             wave_fft<-1>(row, scratch, twl, s, lane, gs, active);
         }
