@@ -92,7 +92,7 @@ def tune(K, nt, batch, cache):
         TUNED.update(K.tune_load(cache))
     ms = None
     if nt not in TUNED:
-        ms = K.autotune(nt, batch, steps=5, cache_path=cache)
+        ms = K.autotune(nt, batch, steps=10, cache_path=cache)
         TUNED.add(nt)
     return dict(zip(["adv_rows", "adv_cols", "fft_rows"], K.tune_get(nt))), ms
 
@@ -105,7 +105,9 @@ def run_uneven(K, steps, cache, sims=64, seed=3):
     sizes = rng.choice([32, 48, 64, 96, 128, 192, 256], sims)
     tiles = {}
     for nt in sorted(set(int(n) for n in sizes)):
-        tiles[nt], _ = tune(K, nt, max(1, int((sizes == nt).sum())), cache)
+        # timed on at least 64 spheres of the size: a handful of small spheres
+        # is launch latency, and its timings pick tiles at random
+        tiles[nt], _ = tune(K, nt, max(64, int((sizes == nt).sum())), cache)
     log(f"uneven tiles: {tiles}")
     with K.Engine(use_graph=True) as eng:
         for k, nt in enumerate(sizes):

@@ -235,6 +235,22 @@ def test_nan_quarantines_only_that_simulation():
         assert s.status() & 1
 
 
+def test_overflowed_velocities_finish_and_are_flagged():
+    """Infinite u_phi and huge u_theta in a band of rows run through every
+    kernel (checked sampler, the cubic's FP64 branch, the transforms) in
+    bounded time and raise the watchdog word."""
+    r = make_request(64, 9)
+    st = [np.array(x, copy=True) for x in r["state"]]
+    st[0][20:28, :] = np.inf
+    st[1][20:28, :] = -1e30
+    with K.Sim(64, 0.01) as s:
+        s.set_fields(*st)
+        s.step(2)
+        s.sync()
+        assert s.status() & 1
+    log("[watchdog] infinite velocities: steps completed and flagged")
+
+
 def test_autotune_runs_and_tile_shapes_never_change_values(tmp_path):
     r = make_request(96, 5)
     base = alone(96, r["dt"], r["state"], 4)
