@@ -367,6 +367,14 @@ __device__ __forceinline__ float advect_node_fast(const float* __restrict__ S, i
     return sample_fast<SW>(bs, fmaf(-(u0p + ump), hP, sP), fmaf(-(u0t + umt), hT, sT));
 }
 
+// Unsigned maximum over the 64 lanes of a wavefront.
+__device__ __forceinline__ unsigned wave_max(unsigned v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, m, 64));
+    return v;
+}
+constexpr int kAdvWaves = 256 / 64;  // wavefronts of an advection workgroup
+
 // Displacement budgets of the fast path, in cells.  Node columns sit at
 // staged columns kEC-1 .. W+kEC and sample offsets lie within half a cell,
 // so phi departures below 2.5 cells keep the stencil in [0, W + 2 kEC); node
@@ -430,9 +438,9 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         st.rlo[K_VPHI] = st.rlo[K_VTHETA] = st.rlo[K_CENTERED] = max(0, 1 - js);
         st.rhi[K_VPHI] = st.rhi[K_CENTERED] = min(SR - 1, nT - 1 - js);
         st.rhi[K_VTHETA] = min(SR - 1, nT - 2 - js);
-        // max |u_phi|, |u_theta| of the staged block and max half phi
-        // coefficient of the tile's node rows (float bits)
-        __shared__ unsigned umax[3];
+        // per wavefront: max |u_phi|, |u_theta| of its staged items and max
+        // half phi coefficient of its node rows (float bits)
+        __shared__ unsigned wmax[3][kAdvWaves];
         // half phi coefficients of node rows j0-1 .. j0+R (u_phi / rho nodes,
         // u_theta nodes) and the geometric coefficient G of those rows, after
         // the staged planes and the advected u block
@@ -442,8 +450,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         cofRow[0] = lds + 2 * plane + max(plane, uvN);
         cofRow[1] = cofRow[0] + (R + 2);
         cofRow[2] = cofRow[1] + (R + 2);
-        if (tid < 3) umax[tid] = 0u;
-        __syncthreads();
+        unsigned mc = 0u;
         if (tid < R + 2) {  // node rows j0-1 .. j0+R, one per thread
             const int j = j0 - 1 + tid;
             const bool in = j >= 0 && j < nT;
@@ -452,7 +459,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
             cofRow[1][tid] = ct;
             cofRow[2][tid] = in ? s.rowf[RT_G * nT + j] : 0.f;
             const float cm = fmaxf(cc, ct);
-            atomicMax(&umax[2], __float_as_uint(cm != cm ? INFINITY : cm));
+            mc = __float_as_uint(cm != cm ? INFINITY : cm);
         }
         // running max of |u| as bits: non-negative floats (and NaN / Inf,
         // above every finite value) order as unsigned, so a non-finite value
@@ -511,14 +518,25 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
                 mt = max(mt, absbits(b));
             }
         }
-        atomicMax(&umax[0], mp);
-        atomicMax(&umax[1], mt);
+        mp = wave_max(mp);
+        mt = wave_max(mt);
+        mc = wave_max(mc);
+        if ((tid & 63) == 0) {
+            wmax[0][tid >> 6] = mp;
+            wmax[1][tid >> 6] = mt;
+            wmax[2][tid >> 6] = mc;
+        }
         __syncthreads();
         // fast tile: every sample an ordinary bilinear read of the staged block
         bool fast;
         {
-            const float Up = __uint_as_float(umax[0]), Ut = __uint_as_float(umax[1]);
-            const float hmax = __uint_as_float(umax[2]);
+            unsigned m[3] = {0u, 0u, 0u};
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int w = 0; w < kAdvWaves; ++w) m[k] = max(m[k], wmax[k][w]);
+            const float Up = __uint_as_float(m[0]), Ut = __uint_as_float(m[1]);
+            const float hmax = __uint_as_float(m[2]);
             // a departure moves at most 2 h max|u| cells
             fast = (j0 >= kER + 1) && (j0 + R <= nT - kER - 1) && (Up * hmax <= 0.5f * kFastDPhi) &&
                    (Ut * hT <= 0.5f * kFastDTheta);

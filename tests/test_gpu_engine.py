@@ -42,13 +42,15 @@ def test_uneven_batch_equals_alone_bitwise():
 
 
 def test_mixed_tile_widths_equal_alone_bitwise():
-    # 256 tuned to 128-column tiles, the others to 64: alone, 256 stages at the
-    # wide stride; in the batch it is restaged as 64-column tiles
+    # 256 tuned to 128-column tiles, 128 to 64 and 64 to 32: alone, each stages
+    # at its own stride; in the batch 256 is restaged as 64-column tiles and
+    # every tile is staged at the 64-column stride
     sizes = [256, 128, 64]
     reqs = [make_request(nt, 300 + k) for k, nt in enumerate(sizes)]
     try:
         K.tune_set(256, 8, 128, 4)
         K.tune_set(128, 16, 64, 4)
+        K.tune_set(64, 16, 32, 4)
         with K.Engine() as eng:
             ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=4) for r in reqs]
             eng.step(4)
@@ -59,7 +61,8 @@ def test_mixed_tile_widths_equal_alone_bitwise():
     finally:
         K.tune_set(256, 0, 0, 0)
         K.tune_set(128, 0, 0, 0)
-    log("[engine] batch mixing 128- and 64-column tiles bitwise equal to alone")
+        K.tune_set(64, 0, 0, 0)
+    log("[engine] batch mixing 128-, 64- and 32-column tiles bitwise equal to alone")
 
 
 def test_graph_and_direct_launches_agree():
@@ -260,7 +263,7 @@ def test_autotune_runs_and_tile_shapes_never_change_values(tmp_path):
     assert open(cache).read().startswith("gfx950 96 4 ")
     tuned = K.tune_get(96)
     try:
-        for shape in [(4, 64, 2), (16, 96, 1), tuned]:
+        for shape in [(4, 64, 2), (16, 96, 1), (32, 32, 2), tuned]:
             K.tune_set(96, *shape)
             got = alone(96, r["dt"], r["state"], 4)
             with K.Engine() as eng:
