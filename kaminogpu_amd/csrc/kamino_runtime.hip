@@ -432,9 +432,10 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
 static const int kGridCap = 1 << 20;
 
 int DeviceBatch::launch_tri(hipStream_t st, bool resid) {
-    // one launch per (L, K) class, in stream order (side streams would let small
-    // classes overlap, but their fork/join graphs cost far more to capture each
-    // time a serving engine's membership changes)
+    // one launch per class (all K = 8 classes of a mixed batch form one, L = 0),
+    // in stream order (side streams would let small classes overlap, but their
+    // fork/join graphs cost far more to capture each time a serving engine's
+    // membership changes)
     for (const TriClass& c : tri_)
         KAM_HIP(launch_tridiag(c.L, c.K, dSims_, dTiles_ + c.off, c.count, kGridCap, st, resid));
     return KAM_OK;
