@@ -54,6 +54,9 @@ struct SimDesc {
     const float* twN;      // float2 [nPhi]: exp(-2 pi i n / nPhi)
     int nfac;              // radix factors of M, innermost first
     int fac[12];
+    int twStep[12];        // per pass p: twiddle index stride nPhi / (Ls r), Ls = fac[0] ... fac[p-1]
+    unsigned lsMagic[12];  // per pass p: ceil(2^32 / Ls) (0 when Ls is a power of two)
+    int fftShift;          // log2 of the lanes per row of the row FFT
     int advRows;           // rows per advection tile
     int fftRows;           // rows per FFT tile
     unsigned magicP;       // ceil(2^32 / nPhi): i / nPhi == umulhi(i, magicP) for i < 2^16

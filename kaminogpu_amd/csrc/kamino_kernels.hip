@@ -787,7 +787,7 @@ __host__ __device__ inline int fft_group(int M) {
 
 template <int RADIX, int DIR>
 __device__ __forceinline__ void wave_pass(const float2* __restrict__ src, float2* __restrict__ dst,
-                                          int M, int Ls, int lsShift,
+                                          int M, int Ls, int lsShift, unsigned lsMagic,
                                           const float2* __restrict__ tw, int twScale, int lane,
                                           int gs) {
     const int nb = M / RADIX;
@@ -797,7 +797,7 @@ __device__ __forceinline__ void wave_pass(const float2* __restrict__ src, float2
             k = b >> lsShift;
             j = b & (Ls - 1);
         } else {
-            k = b / Ls;
+            k = (int)__umulhi((unsigned)b, lsMagic);  // b / Ls, exact for b < 2^16
             j = b - k * Ls;
         }
         const float2* in = src + j + k * Ls;
@@ -833,14 +833,15 @@ __device__ void wave_fft(float2* row, float2* scratch, const float2* tw, const S
     int Ls = 1;
     for (int p = 0; p < s.nfac; ++p) {
         const int r = s.fac[p];
-        const int twScale = s.nPhi / (Ls * r);
+        const int twScale = s.twStep[p];  // nPhi / (Ls r)
+        const unsigned lsMagic = s.lsMagic[p];
         const int lsShift = ((Ls & (Ls - 1)) == 0) ? __builtin_ctz(Ls) : -1;
         if (active) {
             switch (r) {
-                case 2: wave_pass<2, DIR>(src, dst, M, Ls, lsShift, tw, twScale, lane, gs); break;
-                case 3: wave_pass<3, DIR>(src, dst, M, Ls, lsShift, tw, twScale, lane, gs); break;
-                case 4: wave_pass<4, DIR>(src, dst, M, Ls, lsShift, tw, twScale, lane, gs); break;
-                default: wave_pass<5, DIR>(src, dst, M, Ls, lsShift, tw, twScale, lane, gs); break;
+                case 2: wave_pass<2, DIR>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
+                case 3: wave_pass<3, DIR>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
+                case 4: wave_pass<4, DIR>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
+                default: wave_pass<5, DIR>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
             }
         }
         wave_sync();
@@ -862,6 +863,9 @@ __device__ void wave_fft(float2* row, float2* scratch, const float2* tw, const S
 constexpr int kDivBatch = 4;   // row points per lane in k_div_fft
 constexpr int kSpecBatch = 8;  // spectrum entries per thread in k_ifft_grad
 constexpr int kGradBatch = 4;  // column pairs per thread in k_ifft_grad
+
+// Workgroup size of the FFT kernels (their launches use exactly this).
+constexpr int kFftBlockShift = 8, kFftBlock = 1 << kFftBlockShift;
 
 // Stages the nPhi twiddles of s into LDS, two per float4 (nPhi is even).
 __device__ __forceinline__ void stage_twiddles(float2* twl, const SimDesc& s, int tid, int nthr) {
@@ -886,7 +890,7 @@ __device__ __forceinline__ void stage_twiddles(float2* twl, const SimDesc& s, in
 // (d-hat_0, d-hat_Nyquist).  This is the reference's cuFFT CUFFT_INVERSE
 // call plus shiftFKernel (:640-656): d-hat_n = (1/nPhi) sum_i d_i e^{-i n i h}.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sims,
+__global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict__ sims,
                                                  const Tile* __restrict__ tiles, int ntiles,
                                                  int parity) {
     extern __shared__ __attribute__((aligned(16))) float2 zlds[];
@@ -896,7 +900,9 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
         const SimDesc& s = sims[tile.sim];
         const int nT = s.nTheta, nP = s.nPhi, M = s.M;
         const int j0 = tile.row0, R = tile.rows;
-        const int gs = fft_group(M), grp = tid / gs, ngrp = nthr / gs, lane = tid - grp * gs;
+        const int gsh = s.fftShift, gs = 1 << gsh;  // lanes per row (a power of two)
+        // kFftBlock threads: kFftBlock / gs lane groups (a power of two)
+        const int ngsh = kFftBlockShift - gsh, grp = tid >> gsh, ngrp = 1 << ngsh, lane = tid & (gs - 1);
         const float* __restrict__ up = s.state[parity][F_UPHI];
         const float* __restrict__ ut = s.state[parity][F_UTHETA];
         const float* __restrict__ divS = s.rowf + RT_DIVS * nT;
@@ -906,7 +912,7 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
         float2* zbuf = zlds + nP;                                  // R rows of M
         float2* scratch = zbuf + (size_t)(R + grp) * M;            // this lane group's row
         const Rsrc upR = make_rsrc(up, nT * nP), utR = make_rsrc(ut, (nT - 1) * nP);
-        const int nPass = (R + ngrp - 1) / ngrp;                   // uniform trip count
+        const int nPass = (R + ngrp - 1) >> ngsh;                  // uniform trip count
         for (int it = 0; it < nPass; ++it) {
             const int r = it * ngrp + grp;
             const bool active = r < R;
@@ -946,7 +952,8 @@ __global__ __launch_bounds__(256) void k_div_fft(const SimDesc* __restrict__ sim
         float2* __restrict__ spec = reinterpret_cast<float2*>(s.spec);
         const float2* tw = twl;
         const float scale = 1.0f / (float)nP;
-        const unsigned magicR = (unsigned)((((uint64_t)1 << 32) + R - 1) / R);
+        // ceil(2^32 / R) in its 32-bit form (R = 1 splits nothing)
+        const unsigned magicR = (R == 1) ? 0u : 0xffffffffu / (unsigned)R + 1u;
         for (int idx = tid; idx < R * M; idx += nthr) {
             const int n = (R == 1) ? idx : (int)__umulhi((unsigned)idx, magicR), r = idx - n * R, j = j0 + r;
             const float2 Zn = zbuf[(size_t)r * M + n];
@@ -1318,7 +1325,7 @@ __global__ __launch_bounds__(256) void k_tridiag_mixed(const SimDesc* __restrict
 // n = 0 mode dropped), then the gradient subtraction of applyPressureTheta
 // (:705-722) and applyPressurePhi (:724-747), in place.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ sims,
+__global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restrict__ sims,
                                                    const Tile* __restrict__ tiles, int ntiles,
                                                    int parity) {
     extern __shared__ __attribute__((aligned(16))) float2 zlds[];
@@ -1329,7 +1336,9 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
         const int nT = s.nTheta, nP = s.nPhi, M = s.M;
         const int j0 = tile.row0, R = tile.rows;
         const int Rp = min(R + 1, nT - j0);  // rows of p needed
-        const int gs = fft_group(M), grp = tid / gs, ngrp = nthr / gs, lane = tid - grp * gs;
+        const int gsh = s.fftShift, gs = 1 << gsh;  // lanes per row (a power of two)
+        // kFftBlock threads: kFftBlock / gs lane groups (a power of two)
+        const int ngsh = kFftBlockShift - gsh, grp = tid >> gsh, ngrp = 1 << ngsh, lane = tid & (gs - 1);
         float2* tw = zlds;                                          // nPhi twiddles
         float2* zbuf = zlds + nP;                                   // Rp rows of M
         float2* scratch = zbuf + (size_t)(Rp + grp) * M;            // this lane group's row
@@ -1368,7 +1377,7 @@ __global__ __launch_bounds__(256) void k_ifft_grad(const SimDesc* __restrict__ s
             }
         }
         __syncthreads();
-        const int nPass = (Rp + ngrp - 1) / ngrp;
+        const int nPass = (Rp + ngrp - 1) >> ngsh;
         for (int it = 0; it < nPass; ++it) {
             const int r = it * ngrp + grp;
             wave_fft<+1>(zbuf + (size_t)(r < Rp ? r : 0) * M, scratch, tw, s, lane, gs, r < Rp);
@@ -1455,7 +1464,7 @@ hipError_t launch_particles(const SimDesc* sims, const Tile* tiles, int ntiles, 
 hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
                           size_t ldsBytes, int gridCap, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_div_fft, dim3(grid_for(ntiles, gridCap)), dim3(256), ldsBytes, st, sims,
+    hipLaunchKernelGGL(k_div_fft, dim3(grid_for(ntiles, gridCap)), dim3(kFftBlock), ldsBytes, st, sims,
                        tiles, ntiles, parity);
     return hipGetLastError();
 }
@@ -1491,7 +1500,7 @@ hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const
 hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
                             size_t ldsBytes, int gridCap, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ifft_grad, dim3(grid_for(ntiles, gridCap)), dim3(256), ldsBytes, st, sims,
+    hipLaunchKernelGGL(k_ifft_grad, dim3(grid_for(ntiles, gridCap)), dim3(kFftBlock), ldsBytes, st, sims,
                        tiles, ntiles, parity);
     return hipGetLastError();
 }

@@ -245,7 +245,14 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
     d.rowd = t.rowd;
     d.twN = t.twN;
     d.nfac = p.nfac;
-    for (int k = 0; k < p.nfac; ++k) d.fac[k] = p.fac[k];
+    int Ls = 1;
+    for (int k = 0; k < p.nfac; ++k) {
+        d.fac[k] = p.fac[k];
+        d.twStep[k] = p.nP / (Ls * p.fac[k]);
+        d.lsMagic[k] = (Ls & (Ls - 1)) == 0 ? 0u : (unsigned)((((uint64_t)1 << 32) + Ls - 1) / Ls);
+        Ls *= p.fac[k];
+    }
+    d.fftShift = __builtin_ctz((unsigned)fft_group_host(p.M));
     d.advRows = p.advRows;
     d.fftRows = p.fftRows;
     d.magicP = (unsigned)((((uint64_t)1 << 32) + (uint64_t)p.nP - 1) / (uint64_t)p.nP);

@@ -4,9 +4,11 @@ import numpy as np
 
 def test_magic_division_exact_for_every_divisor_used():
     # Flat item loops split an index as q = umulhi(i, ceil(2^32 / d)): tile
-    # widths W, W+1, W+2 (k_advect), K = rows per lane and nTheta (k_tridiag
-    # staging); every divisor 2..2048 (k_tridiag handles K = 1 apart: its
-    # multiplier 2^32 does not fit 32 bits), every dividend below 2^16.
+    # widths W, W+1, W+2 and the staged width (k_advect), K = rows per lane
+    # (k_tridiag staging), rows per FFT tile, the Stockham stride Ls of a
+    # row-FFT pass when it is not a power of two; every divisor 2..2048
+    # (K = 1 and R = 1 are taken apart: their multiplier 2^32 does not fit
+    # 32 bits), every dividend below 2^16.
     idx = np.arange(1 << 16, dtype=np.uint64)
     for d in range(2, 2049):
         m = ((1 << 32) + d - 1) // d
