@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--uneven-steps", type=int, default=200, help="config-3 batch steps (0 skips)")
     ap.add_argument("--large-steps", type=int, default=500, help="config-2 steps at 1024x2048 (0 skips)")
     ap.add_argument("--serve-requests", type=int, default=10000, help="config-4 requests (0 skips)")
+    ap.add_argument("--launch", choices=["graph", "direct"], default="direct",
+                    help="engine steps as direct kernel launches (default: each hipGraph launch "
+                         "boundary idles the GPU ~10 us) or as hipGraph replays")
     ap.add_argument("--tune-cache", default=None,
                     help="tuning cache: sizes it holds are loaded, the others autotuned and appended")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -82,6 +85,7 @@ def cpu_baseline(nt, dt, seconds):
 
 
 TUNED = set()
+USE_GRAPH = False  # set from --launch
 
 
 def tune(K, nt, batch, cache):
@@ -109,7 +113,7 @@ def run_uneven(K, steps, cache, sims=64, seed=3):
         # is launch latency, and its timings pick tiles at random
         tiles[nt], _ = tune(K, nt, max(64, int((sizes == nt).sum())), cache)
     log(f"uneven tiles: {tiles}")
-    with K.Engine(use_graph=True) as eng:
+    with K.Engine(use_graph=USE_GRAPH) as eng:
         for k, nt in enumerate(sizes):
             st = K.init_fields(int(nt), min(16, int(nt) // 4), 100 + k, K.DENSITY_BAND)
             eng.admit(int(nt), 0.5 / nt, *st, steps=steps + 20)
@@ -136,7 +140,7 @@ def run_served(K, n):
     reqs = serve.make_requests(n, seed=4)
     work = float(np.mean([2 * r.n_theta ** 2 * r.steps for r in reqs]))
     pool = serve.StatePool()
-    with K.Engine(use_graph=True) as eng:
+    with K.Engine(use_graph=USE_GRAPH) as eng:
         cap = serve.capacity(serve.make_requests(128, seed=5), eng, pool)
         rate = 0.8 * cap / work
         log(f"serve: mix capacity {cap:.3e} cell-steps/s -> {rate:.1f} requests/s")
@@ -201,6 +205,8 @@ def copy_calibration(nbytes):
 
 def main():
     a = parse()
+    global USE_GRAPH
+    USE_GRAPH = a.launch == "graph"
     if a.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has one MI355X; "
               "multi-GPU sharding is design-only.", file=sys.stderr)
@@ -229,10 +235,10 @@ def main():
     tiles, tuned_ms = tune(K, nt, B, a.tune_cache)
     log(f"tiles {nt}: {tiles}" + ("" if tuned_ms is None else f", autotuned, kernel ms {np.round(tuned_ms, 4)}"))
 
-    # ---- batched form: B spheres through the engine, graph-replayed steps
+    # ---- batched form: B spheres through the engine (graph-replayed or directly launched steps)
     states = [init_state(nt, 16, 1 + b, 1) for b in range(min(B, 8))]
     total_steps = a.warmup + a.steps + a.profile_steps + 10
-    with K.Engine(use_graph=True) as eng:
+    with K.Engine(use_graph=USE_GRAPH) as eng:
         for b in range(B):
             if eng.admit(nt, dt, *states[b % len(states)], steps=total_steps) is None:
                 raise SystemExit("engine refused the batch")
@@ -275,7 +281,7 @@ def main():
         "data": "synthetic (seeded spherical-harmonic streamfunction, density band)",
         "config": {"workload": f"batched: {B} spheres of {nt}x{2 * nt}, dt {dt}, one engine step "
                                f"advances all {B}", "n_theta": nt, "n_phi": 2 * nt, "sims": B,
-                   "tiles": tiles},
+                   "tiles": tiles, "launch": a.launch},
         "batched": {"cell_steps_per_s": batched, "p50_step_ms": st.step_p50_ms,
                     "p99_step_ms": st.step_p99_ms, "speedup_vs_plain": batched / plain if plain else None},
         "plain": {"cell_steps_per_s": plain, "steps": a.plain_steps,

@@ -226,6 +226,9 @@ int rebuild(kam_engine* e) {
     return KAM_OK;
 }
 
+// Step events time the steps and pace the host; no host read of device
+// memory waits on them (result copies and the watchdog snapshot have events of
+// their own), so they skip the system-scope release fence.
 hipEvent_t take_event(kam_engine* e) {
     if (!e->eventPool.empty()) {
         hipEvent_t ev = e->eventPool.back();
@@ -233,7 +236,7 @@ hipEvent_t take_event(kam_engine* e) {
         return ev;
     }
     hipEvent_t ev = nullptr;
-    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableSystemFence) != hipSuccess) return nullptr;
     return ev;
 }
 

@@ -56,9 +56,12 @@ def bind(L):
 
 
 class Engine:
-    """Batched / served stepping of many simulations on one GPU."""
+    """Batched / served stepping of many simulations on one GPU.  Steps go out
+    as direct kernel launches by default; use_graph=True replays each step as
+    a hipGraph instead (same results; each graph launch boundary costs the GPU
+    ~10 us, more than the launches it saves for steps of 0.1 ms and up)."""
 
-    def __init__(self, device=0, max_slots=0, hbm_budget_bytes=0, use_graph=True, max_inflight=0):
+    def __init__(self, device=0, max_slots=0, hbm_budget_bytes=0, use_graph=False, max_inflight=0):
         from . import check, lib
 
         self._lib = lib()

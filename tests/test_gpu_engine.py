@@ -93,9 +93,11 @@ def test_one_batched_step_matches_oracle():
             assert max(errs) < TOL_FIELD
 
 
-def test_serving_mix_continuous_admission():
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_serving_mix_continuous_admission(use_graph):
     """Requests arrive while others run; each is admitted as a slot frees up,
-    runs its own step count inside shared steps, and equals its solo run."""
+    runs its own step count inside shared steps, and equals its solo run
+    (with graph replay, every membership change re-captures the step)."""
     rng = np.random.default_rng(5)
     pending = []
     for k in range(10):
@@ -106,7 +108,7 @@ def test_serving_mix_continuous_admission():
         pending.append(r)
     pending.sort(key=lambda r: r["arrive"])
     done = {}
-    with K.Engine(max_slots=3) as eng:
+    with K.Engine(max_slots=3, use_graph=use_graph) as eng:
         running = {}
         t = 0
         while pending or running:
