@@ -203,6 +203,34 @@ def copy_calibration(nbytes):
     log(f"copy calibration: {nbytes} bytes filled, then copied device to device")
 
 
+def copy_bandwidth(nbytes=1 << 30, reps=10):
+    """Streaming bandwidth of this GPU as HIP measures it: device-to-device
+    copies of nbytes (read + write), timed with HIP events after one warm-up."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    n = C.c_size_t(nbytes)
+    assert hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    assert hip.hipMemset(a, 1, n) == 0
+    assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice, warm-up
+    assert hip.hipEventRecord(e0, None) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpyAsync(b, a, n, 3, None) == 0
+    assert hip.hipEventRecord(e1, None) == 0 and hip.hipEventSynchronize(e1) == 0
+    ms = C.c_float()
+    assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+    for ev in (e0, e1):
+        hip.hipEventDestroy(ev)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    gbs = 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+    log(f"streaming copy: {gbs:.0f} GB/s ({reps} device-to-device copies of {nbytes >> 20} MiB)")
+    return gbs
+
+
 def main():
     a = parse()
     global USE_GRAPH
@@ -219,6 +247,7 @@ def main():
     cells = nt * 2 * nt
     if a.copy_calibration_mib > 0:
         copy_calibration(a.copy_calibration_mib << 20)
+    copy_gbs = copy_bandwidth()
 
     # ---- plain form: one simulation, per-step kernel sequence, no graph
     plain, p_steps = None, np.zeros(1)
@@ -294,6 +323,7 @@ def main():
         "kernel_gbs": gbs,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "copy_gbs": copy_gbs, "frac_of_copy": achieved / copy_gbs,
                      "algorithmic_bytes": BYTES_PER_CELL[dom] * cells * B,
                      "valu_issue_frac": valu_frac},
     }
