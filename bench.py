@@ -155,10 +155,13 @@ def run_served(K, n):
     return out
 
 
-def run_large(K, steps, nt=1024):
+def run_large(K, steps, cache, nt=1024):
     """Config 2: one 1024x2048 sphere, dt = 0.5/nTheta, seeded flow with L = 32,
-    stepped alone (plain launches, FP32 fields, FP64 tridiagonal)."""
+    stepped alone (plain launches, FP32 fields, FP64 tridiagonal); tiles
+    autotuned on the sphere itself."""
     dt = 0.5 / nt
+    tiles, _ = tune(K, nt, 1, cache)
+    log(f"large tiles: {tiles}")
     with K.Sim(nt, dt) as s:
         s.set_fields(*init_state(nt, 32, 2, 1))
         s.benchmark(10)
@@ -167,7 +170,7 @@ def run_large(K, steps, nt=1024):
     v = nt * 2 * nt * steps / (total * 1e-3)
     log(f"large {nt}x{2 * nt}: {v:.3e} cell-steps/s, {total / steps:.4f} ms/step, kernel ms {np.round(kern, 4)}")
     return {"n_theta": nt, "steps": steps, "cell_steps_per_s": v, "ms_per_step": total / steps,
-            "p99_step_ms": float(np.percentile(per, 99)),
+            "p99_step_ms": float(np.percentile(per, 99)), "tiles": tiles,
             "kernel_ms": dict(zip(KERNELS, [float(x) for x in kern]))}
 
 
@@ -281,7 +284,7 @@ def main():
     log(f"batched x{B}: {batched:.3e} cell-steps/s, {ms_per_step:.4f} ms/step, "
         f"p50 {st.step_p50_ms:.4f} p99 {st.step_p99_ms:.4f}")
 
-    large = run_large(K, a.large_steps) if a.large_steps > 0 else None
+    large = run_large(K, a.large_steps, a.tune_cache) if a.large_steps > 0 else None
     uneven = run_uneven(K, a.uneven_steps, a.tune_cache) if a.uneven_steps > 0 else None
     served = run_served(K, a.serve_requests) if a.serve_requests > 0 else None
 

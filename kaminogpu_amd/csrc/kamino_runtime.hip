@@ -118,6 +118,10 @@ int make_plan(int nT, int nP, Plan& p) {
     // FFT tile: at most 8 rows (+1 halo row for the inverse, + scratch rows) in ~48 KiB
     int Rf = (int)(49152 / (8 * (size_t)p.M)) - 3 - 256 / fft_group_host(p.M);
     Rf = Rf < 1 ? 1 : (Rf > 8 ? 8 : Rf);
+    // long rows (M = 1024: 16 KiB of twiddles alone) still get two rows per
+    // tile, in 72 KiB: with one, three of the four lane groups sit idle
+    // (measured at 1024x2048: both FFT kernels 0.069 -> 0.046 ms)
+    if (Rf < 2 && fft_lds(2, p.M) <= 160 * 1024) Rf = 2;
     p.fftRows = Rf > nT ? nT : Rf;
     // tridiagonal: L lanes per mode, at most K rows per lane
     if (nT <= 64) { p.triL = 8; p.triK = 8; }
