@@ -110,9 +110,10 @@ int make_plan(int nT, int nP, Plan& p) {
     if (p.nfac < 0)
         return set_err(KAM_EINVAL, "n_theta must factor as 2^a 3^b 5^c (got %d)", nT);
     // advection tile: 16 rows x 64 columns (8 rows below nTheta = 128), about
-    // 30 KiB of LDS, the shape the autotuner keeps picking; narrower spheres
-    // use their full width.  A batch stages every tile at the widest member's
-    // row stride, so one common default width keeps mixed batches compact.
+    // 22 KiB of LDS, the shape the autotuner picks for most sizes of a mixed
+    // batch; narrower spheres use their full width.  A batch stages every tile
+    // at the widest member's row stride, so one common default width keeps
+    // mixed batches compact.
     p.advCols = nP < 64 ? nP : 64;
     p.advRows = nT < 128 ? (nT < 8 ? nT : 8) : 16;
     // FFT tile: at most 8 rows (+1 halo row for the inverse, + scratch rows) in ~48 KiB
