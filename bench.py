@@ -103,7 +103,7 @@ def tune(K, nt, batch, cache):
 
 def run_uneven(K, steps, cache, sims=64, seed=3):
     """Config 3: 64 spheres with n_theta drawn from {32,...,256}, dt = 0.5/n_theta,
-    L = min(16, n_theta/4), density band, stepped together as graph replays;
+    L = min(16, n_theta/4), density band, stepped together through one engine;
     tiles of each size from the autotuner."""
     rng = np.random.default_rng(seed)
     sizes = rng.choice([32, 48, 64, 96, 128, 192, 256], sims)
