@@ -591,7 +591,38 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         const int nU = (uHi - uLo + 1) * W2, nV = (vHi - vLo + 1) * W1, nR = R * W;
         // one backtrace per item: the loops end at wavefront granularity, with
         // no duplicated tail items (occupancy hides the LDS latency)
+        // Tiles 64 or 128 columns wide map each thread to one tile column
+        // (cm) and a run of rows (rm0, rm0 + rmS, ...), with no index split
+        // per item; the u blocks' edge columns (u_phi at -1 and W, u_theta at
+        // -1) then take one short flat pass, from the last wavefront down.
+        // Other widths split a flat item index.  The mapping never changes a value.
+        const bool colMap = (W == 64 || W == 128);
+        const int cm = tid & (W - 1), rmS = nthr >> (W == 128 ? 7 : 6), rm0 = tid >> (W == 128 ? 7 : 6);
+        const int nUr = uHi - uLo + 1, nVr = vHi - vLo + 1;
         auto advect_all = [&](auto F) {
+            if (colMap) {
+                for (int r = rm0; r < R; r += rmS) {
+                    const int j = j0 + r;
+                    const float a = node(F, KRho{}, cm, j, cofRow[0][r + 1]);
+                    orh[j * nP + c0 + cm] = a;
+                    flag_bad(s.status, !isfinite(a));
+                }
+                __syncthreads();  // the staged rho plane is dead from here on
+                for (int r = rm0; r < nUr; r += rmS)
+                    U[__mul24(r, W2) + cm + 1] = node(F, KPhi{}, cm, uLo + r, cofRow[0][uLo + r - j0 + 1]);
+                for (int r = rm0; r < nVr; r += rmS)
+                    V[__mul24(r, W1) + cm + 1] = node(F, KTheta{}, cm, vLo + r, cofRow[1][vLo + r - j0 + 1]);
+                for (int idx = nthr - 1 - tid; idx < 2 * nUr + nVr; idx += nthr) {
+                    if (idx < 2 * nUr) {
+                        const int r = idx >> 1, c = (idx & 1) ? W : -1;
+                        U[__mul24(r, W2) + c + 1] = node(F, KPhi{}, c, uLo + r, cofRow[0][uLo + r - j0 + 1]);
+                    } else {
+                        const int r = idx - 2 * nUr;
+                        V[__mul24(r, W1)] = node(F, KTheta{}, -1, vLo + r, cofRow[1][vLo + r - j0 + 1]);
+                    }
+                }
+                return;
+            }
             for (int idx = tid; idx < nR; idx += nthr) {
                 const int r = (int)__umulhi((unsigned)idx, mW), c = idx - __mul24(r, W), j = j0 + r;
                 const float a = node(F, KRho{}, c, j, cofRow[0][r + 1]);
@@ -631,8 +662,8 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         // restagger), both into the dead staged block
         const int nG = (uHi - j0 + 1) * W1;
         float* VP = X + (R + 1) * W1;  // (R+1) x W1: v'
-        for (int idx = tid; idx < nG; idx += nthr) {
-            const int r = (int)__umulhi((unsigned)idx, mW1), c = idx - r * W1, j = j0 + r;
+        auto geo = [&](int r, int c) {
+            const int idx = r * W1 + c, j = j0 + r;
             const float* Ur = U + (j - uLo) * W2;
             const float ub = 0.5f * (Ur[c] + Ur[c + 1]);
             float vb;
@@ -646,15 +677,32 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
             const float x = geo_root(G, vb, ub);
             X[idx] = x;
             VP[idx] = fmaf(G * x, x, vb);
+        };
+        if (colMap) {
+            const int nGr = uHi - j0 + 1;
+            for (int r = rm0; r < nGr; r += rmS) geo(r, cm);
+            for (int r = nthr - 1 - tid; r < nGr; r += nthr) geo(r, W);  // column W
+        } else {
+            for (int idx = tid; idx < nG; idx += nthr) {
+                const int r = (int)__umulhi((unsigned)idx, mW1);
+                geo(r, idx - r * W1);
+            }
         }
         __syncthreads();
         // assignPhiKernel (rows j0..j0+R-1) and assignThetaKernel (rows
         // j0..uHi-1), columns c0..c0+W-1
-        for (int idx = tid; idx < nR; idx += nthr) {
-            const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
+        auto assign = [&](int r, int c) {
             const int o = (j0 + r) * nP + c0 + c, x = r * W1 + c;
             oup[o] = 0.5f * (X[x] + X[x + 1]);
             if (j0 + r < uHi) out[o] = 0.5f * (VP[x + 1] + VP[x + W1 + 1]);
+        };
+        if (colMap) {
+            for (int r = rm0; r < R; r += rmS) assign(r, cm);
+        } else {
+            for (int idx = tid; idx < nR; idx += nthr) {
+                const int r = (int)__umulhi((unsigned)idx, mW);
+                assign(r, idx - r * W);
+            }
         }
         __syncthreads();
     }
