@@ -351,12 +351,12 @@ __device__ __forceinline__ float sample_node_const(const float* __restrict__ nb)
 }
 
 template <int NODE, int SRC, int SW>
-__device__ __forceinline__ float advect_node_fast(const float* __restrict__ S, int plane, int lc,
-                                                  int lr, float hP, float hT) {
+__device__ __forceinline__ float advect_node_fast(const float* __restrict__ S, int plane, int node,
+                                                  float hP, float hT) {
     constexpr float pP = kNodeP[NODE] + kShiftP[K_VPHI], pT = kNodeT[NODE] + kShiftT[K_VPHI];
     constexpr float tP = kNodeP[NODE] + kShiftP[K_VTHETA], tT = kNodeT[NODE] + kShiftT[K_VTHETA];
     constexpr float sP = kNodeP[NODE] + kShiftP[SRC], sT = kNodeT[NODE] + kShiftT[SRC];
-    const int o = opaque(__mul24(lr, SW) + lc);
+    const int o = opaque(node);             // the node's staged offset, row * SW + column
     const float* bp = S + o;                // node in the u_phi plane
     const float* bt = S + (o + plane);      // ... u_theta plane
     const float* bs = S + (o + SRC * plane);  // ... source field plane
@@ -543,16 +543,20 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         }
 
         // one backtrace at node (c, j) of the tile (c from -1, j global): the
-        // branch-free staged sampler on fast tiles, the checked one otherwise
-        auto node = [&](auto fastTag, auto nodeTag, int c, int j, float hP) -> float {
+        // branch-free staged sampler on fast tiles, the checked one otherwise;
+        // ofs = the node's staged offset (j - js) SW + c + kEC, which row walks
+        // carry from one row to the next
+        auto nodeAt = [&](auto fastTag, auto nodeTag, int c, int j, float hP, int ofs) -> float {
             constexpr int NK = decltype(nodeTag)::value;
-            const int lc = c + kEC, lr = j - js;
             if constexpr (decltype(fastTag)::value) {
-                return advect_node_fast<NK, NK, SW>(lds, plane, lc, lr, hP, hT);
+                return advect_node_fast<NK, NK, SW>(lds, plane, ofs, hP, hT);
             } else {
                 const Rsrc src = (NK == K_VPHI) ? up : ((NK == K_VTHETA) ? ut : rh);
-                return advect_node_st<NK, NK, SW>(st, up, ut, src, g, col(c), j, lc, lr, hP, hT);
+                return advect_node_st<NK, NK, SW>(st, up, ut, src, g, col(c), j, c + kEC, j - js, hP, hT);
             }
+        };
+        auto node = [&](auto fastTag, auto nodeTag, int c, int j, float hP) -> float {
+            return nodeAt(fastTag, nodeTag, c, j, hP, __mul24(j - js, SW) + c + kEC);
         };
         using KPhi = std::integral_constant<int, K_VPHI>;
         using KTheta = std::integral_constant<int, K_VTHETA>;
@@ -601,17 +605,23 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         const int nUr = uHi - uLo + 1, nVr = vHi - vLo + 1;
         auto advect_all = [&](auto F) {
             if (colMap) {
-                for (int r = rm0; r < R; r += rmS) {
-                    const int j = j0 + r;
-                    const float a = node(F, KRho{}, cm, j, cofRow[0][r + 1]);
-                    orh[j * nP + c0 + cm] = a;
+                const int dOfs = __mul24(rmS, SW);
+                int ofs = __mul24(j0 + rm0 - js, SW) + cm + kEC;
+                float* o = orh + (j0 + rm0) * nP + c0 + cm;
+                for (int r = rm0; r < R; r += rmS, ofs += dOfs, o += rmS * nP) {
+                    const float a = nodeAt(F, KRho{}, cm, j0 + r, cofRow[0][r + 1], ofs);
+                    *o = a;
                     flag_bad(s.status, !isfinite(a));
                 }
                 __syncthreads();  // the staged rho plane is dead from here on
-                for (int r = rm0; r < nUr; r += rmS)
-                    U[__mul24(r, W2) + cm + 1] = node(F, KPhi{}, cm, uLo + r, cofRow[0][uLo + r - j0 + 1]);
-                for (int r = rm0; r < nVr; r += rmS)
-                    V[__mul24(r, W1) + cm + 1] = node(F, KTheta{}, cm, vLo + r, cofRow[1][vLo + r - j0 + 1]);
+                ofs = __mul24(uLo + rm0 - js, SW) + cm + kEC;
+                float* u = U + __mul24(rm0, W2) + cm + 1;
+                for (int r = rm0; r < nUr; r += rmS, ofs += dOfs, u += __mul24(rmS, W2))
+                    *u = nodeAt(F, KPhi{}, cm, uLo + r, cofRow[0][uLo + r - j0 + 1], ofs);
+                ofs = __mul24(vLo + rm0 - js, SW) + cm + kEC;
+                float* v = V + __mul24(rm0, W1) + cm + 1;
+                for (int r = rm0; r < nVr; r += rmS, ofs += dOfs, v += __mul24(rmS, W1))
+                    *v = nodeAt(F, KTheta{}, cm, vLo + r, cofRow[1][vLo + r - j0 + 1], ofs);
                 for (int idx = nthr - 1 - tid; idx < 2 * nUr + nVr; idx += nthr) {
                     if (idx < 2 * nUr) {
                         const int r = idx >> 1, c = (idx & 1) ? W : -1;
