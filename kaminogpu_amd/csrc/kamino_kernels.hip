@@ -593,13 +593,13 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         const bool north = (j0 == 0), south = (uHi == nT - 1);
         const int half = g.half;
         const int nU = (uHi - uLo + 1) * W2, nV = (vHi - vLo + 1) * W1, nR = R * W;
-        // one backtrace per item: the loops end at wavefront granularity, with
-        // no duplicated tail items (occupancy hides the LDS latency)
-        // Tiles 64 or 128 columns wide map each thread to one tile column
-        // (cm) and a run of rows (rm0, rm0 + rmS, ...), with no index split
-        // per item; the u blocks' edge columns (u_phi at -1 and W, u_theta at
-        // -1) then take one short flat pass, from the last wavefront down.
-        // Other widths split a flat item index.  The mapping never changes a value.
+        // One backtrace per item.  Tiles 64 or 128 columns wide map each thread
+        // to one tile column (cm) and a run of rows (rm0, rm0 + rmS, ...), with
+        // no index split per item; the u blocks' edge columns (u_phi at -1 and
+        // W, u_theta at -1) then take one short flat pass, from the last
+        // wavefront down.  Other widths split a flat item index; those loops end
+        // at wavefront granularity, with no duplicated tail items (occupancy
+        // hides the LDS latency).  The mapping never changes a value.
         const bool colMap = (W == 64 || W == 128);
         const int cm = tid & (W - 1), rmS = nthr >> (W == 128 ? 7 : 6), rm0 = tid >> (W == 128 ? 7 : 6);
         const int nUr = uHi - uLo + 1, nVr = vHi - vLo + 1;
