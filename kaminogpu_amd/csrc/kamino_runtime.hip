@@ -367,14 +367,29 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
     shrink_until_busy(plans, advRows, minTiles, advCols.data());
     shrink_until_busy(plans, fftRows, minTiles, nullptr);
     // advection tiles, in batch order (tallest simulations first); the last
-    // column tile of a row band overlaps its neighbour instead of running short
+    // column tile of a row band overlaps its neighbour instead of running short.
+    // A tile starting within kAdvHaloRows + 1 rows of a pole (or ending within
+    // them) samples with the checked sampler throughout, so the two polar bands
+    // are exactly that tall when the tile height allows, and every band between
+    // them can take the fast one; the last inner band may be shorter.
     advOff_ = 0;
+    constexpr int kPoleRows = kAdvHaloRows + 1;
     for (size_t s = 0; s < n; ++s) {
         const Plan& p = plans[s];
-        const int W = advCols[s];
-        for (int j = 0; j < p.nT; j += advRows[s])
+        const int W = advCols[s], R = advRows[s];
+        const bool polar = R >= kPoleRows && p.nT > 2 * kPoleRows;
+        for (int j = 0, rows; j < p.nT; j += rows) {
+            if (!polar)
+                rows = std::min(R, p.nT - j);
+            else if (j == 0)
+                rows = kPoleRows;
+            else if (j >= p.nT - kPoleRows)
+                rows = p.nT - j;
+            else
+                rows = std::min(R, p.nT - kPoleRows - j);
             for (int c = 0; c < p.nP; c += W)
-                hTiles_.push_back(Tile{(int)s, j, std::min(advRows[s], p.nT - j), std::min(c, p.nP - W)});
+                hTiles_.push_back(Tile{(int)s, j, rows, std::min(c, p.nP - W)});
+        }
         advLds_ = std::max(advLds_, adv_lds(advRows[s], W, advStride_, (sims[s].flags & SIM_GEOMETRIC) != 0));
     }
     nAdv_ = (int)hTiles_.size();
