@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libkamino_oracle.so")
+# KAMINO_ORACLE_LIB names another build of the same sources (the mutation tests
+# load deliberately broken copies this way to show that the pins catch them).
+LIB_PATH = os.environ.get("KAMINO_ORACLE_LIB") or os.path.join(HERE, "_build", "libkamino_oracle.so")
 
 VPHI, VTHETA, CENTERED = 0, 1, 2
 PHASE_ADVECT, PHASE_GEOMETRIC, PHASE_PROJECT, PHASE_ALL = 1, 2, 4, 7

@@ -214,14 +214,34 @@ def test_geometric_closed_form_at_dt_zero():
     np.testing.assert_allclose(gt, 0.5 * (vbar[:-1] + vbar[1:]), atol=1e-14)
 
 
-def test_geometric_cubic_residual_at_dt():
-    nt, nph, dt = 16, 32, 0.01
-    up, ut, _ = O.init_fields(nt, nph, 4, 2, 0)
-    gp, gt = O.geometric(nt, nph, dt, up, ut)
-    # the restaggered u_theta equals the average of v' = vbar + G x^2; recover x from u_phi'
-    # is ill-posed, so check the energy-like identity only: outputs are finite and close
-    assert np.isfinite(gp).all() and np.isfinite(gt).all()
-    assert np.abs(gp - 0.25 * (np.roll(up, 1, 1) + 2 * up + np.roll(up, -1, 1))).max() < 0.05
+def test_geometric_zonal_flow_at_dt():
+    """Zonal flow u_phi = f(theta_j), u_theta = 0, at dt > 0 and R != 1: every
+    cell of row j has ubar = f_j and vbar = 0 (also in the pole rows' 3/4, 1/4
+    antipodal mix), so x_j is the one real root of G_j^2 x^3 + x - f_j with
+    G_j = dt cot(theta_j) / R (KaminoCore.cu:494-510), taken here from
+    numpy.roots; u_phi' = x_j (assignPhiKernel :517-534) and
+    u_theta'_j = (G_j x_j^2 + G_{j+1} x_{j+1}^2) / 2 (v' = vbar + G x^2,
+    assignThetaKernel :537-549)."""
+    nt, nph, dt, R = 16, 32, 0.05, 2.0
+    h = np.pi / nt
+    th = (np.arange(nt) + 0.5) * h
+    f = 1.0 + 0.5 * np.cos(th) + 0.3 * np.sin(3 * th)
+    up = np.repeat(f[:, None], nph, 1)
+    ut = np.zeros((nt - 1, nph))
+    gp, gt = O.geometric(nt, nph, dt, up, ut, radius=R)
+    G = dt * np.cos(th) / (R * np.sin(th))
+    x = np.empty(nt)
+    for j in range(nt):
+        if abs(G[j]) <= 1e-7:
+            x[j] = f[j]
+            continue
+        r = np.roots([G[j] ** 2, 0.0, 1.0, -f[j]])
+        x[j] = r[np.argmin(np.abs(r.imag))].real
+    assert np.abs(G).max() > 0.2  # the cubic term matters near the poles
+    np.testing.assert_allclose(gp, np.repeat(x[:, None], nph, 1), rtol=1e-10, atol=1e-12)
+    vp = G * x ** 2
+    np.testing.assert_allclose(gt, np.repeat((0.5 * (vp[:-1] + vp[1:]))[:, None], nph, 1),
+                               rtol=1e-10, atol=1e-12)
 
 
 # ------------------------------------------------------------- samplers ----
@@ -315,6 +335,72 @@ def test_uniform_rotation_by_one_cell_shifts_density():
     _, ot, orr, *_ = O.advect(nt, 2 * nt, dt, up, ut, rho)
     np.testing.assert_allclose(orr, np.roll(rho, 1, 1), atol=1e-12)
     np.testing.assert_array_equal(ot, ut)
+
+
+def _linear_density(nt, alpha, beta, gamma):
+    """rho = alpha + beta theta + gamma phi at the rho nodes (ih, (j + 1/2) h):
+    bilinear interpolation reproduces it exactly away from the phi seam."""
+    h = np.pi / nt
+    th = (np.arange(nt) + 0.5) * h
+    ph = np.arange(2 * nt) * h
+    return alpha + beta * th[:, None] + gamma * ph[None, :], th, ph
+
+
+def test_uniform_flow_departure_point():
+    """Uniform u_theta = c and u_phi = U with R != 1: every velocity sample is
+    (U, c), so the RK2 departure point of the rho node g is
+    (phi_g - U dt / (R sin theta_g), theta_g - c dt / R) (advectionCentered,
+    KaminoCore.cu:292-315: the phi metric stays at theta_g), and a density linear in theta
+    and phi is advected to its value there.  Pins the sign of both
+    displacements and the dt/R and dt/(R sin theta_g) metrics."""
+    nt, dt, R, c, U = 32, 0.01, 2.0, 0.8, 1.5
+    alpha, beta, gamma = 0.3, 0.5, 0.02
+    rho, th, ph = _linear_density(nt, alpha, beta, gamma)
+    up = np.full((nt, 2 * nt), U)
+    ut = np.full((nt - 1, 2 * nt), c)
+    op, ot, orr, *_ = O.advect(nt, 2 * nt, dt, up, ut, rho, radius=R)
+    thp = th - c * dt / R
+    php = ph[None, :] - U * dt / (R * np.sin(th))[:, None]
+    expect = alpha + beta * thp[:, None] + gamma * php
+    rows, cols = slice(2, nt - 2), slice(3, 2 * nt - 3)  # interior stencils, clear of the seam
+    np.testing.assert_allclose(orr[rows, cols], expect[rows, cols], rtol=0, atol=1e-12)
+    # constant velocity fields advect to themselves
+    np.testing.assert_allclose(op[rows], U, rtol=1e-13)
+    np.testing.assert_allclose(ot[rows], c, rtol=1e-13)
+
+
+def test_linear_meridional_flow_rk2_rule():
+    """u_theta = k (theta - theta0), u_phi = 0: at a node theta_g the reference
+    takes u0 = k (theta_g - theta0), the midpoint m = theta_g - M u0 / 2
+    (M = dt / R), u(m) = u0 (1 - k M / 2) and the average of the two,
+    ubar = u0 (1 - k M / 4) (KaminoCore.cu:209-224, 254-268, 299-315), so the departure point
+    is theta_g - M k (theta_g - theta0)(1 - k M / 4); plain midpoint RK2
+    (ubar = u(m)) would give 1 - k M / 2.  Interpolation of the linear fields
+    is exact on interior rows, so this pins the averaging rule for rho (at
+    rho nodes) and for u_theta itself (at u_theta nodes)."""
+    nt, dt, R, k = 32, 0.02, 1.5, 3.0
+    th0 = np.pi / 2
+    h = np.pi / nt
+    M = dt / R
+    alpha, beta = 0.2, 0.7
+    rho, th, _ = _linear_density(nt, alpha, beta, 0.0)
+    tht = (np.arange(nt - 1) + 1.0) * h  # u_theta node latitudes
+    ut = np.repeat((k * (tht - th0))[:, None], 2 * nt, 1)
+    up = np.zeros((nt, 2 * nt))
+    op, ot, orr, *_ = O.advect(nt, 2 * nt, dt, up, ut, rho, radius=R)
+
+    def depart(t):
+        return t - M * k * (t - th0) * (1.0 - k * M / 4.0)
+
+    rows = slice(3, nt - 3)
+    np.testing.assert_allclose(orr[rows], (alpha + beta * depart(th))[rows, None] * np.ones(2 * nt),
+                               rtol=0, atol=1e-12)
+    np.testing.assert_allclose(ot[rows], (k * (depart(tht) - th0))[rows, None] * np.ones(2 * nt),
+                               rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(op, 0.0)
+    # the plain-midpoint value differs by far more than the tolerance
+    plain = th - M * k * (th - th0) * (1.0 - k * M / 2.0)
+    assert np.abs(beta * (plain - depart(th))[rows]).max() > 1e-5
 
 
 def test_ambiguity_mask_is_sparse_for_smooth_flow():
