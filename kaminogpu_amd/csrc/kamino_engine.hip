@@ -589,6 +589,19 @@ int kam_engine_cancel(kam_engine* e, uint64_t id) {
     auto it = e->reqs.find(id);
     if (it == e->reqs.end()) return set_err(KAM_ENOTFOUND, "unknown request %llu", (unsigned long long)id);
     if (it->second.state != KAM_REQ_ACTIVE) {
+        // finished (its final copy may still be in flight), failed or
+        // cancelled: give back its pinned copy and event, and forget it in the
+        // completion list so that no later query reports it
+        Request& r = it->second;
+        if (r.copied) {
+            KAM_HIP(hipEventSynchronize(r.copied));
+            (void)hipEventDestroy(r.copied);
+            r.copied = nullptr;
+        }
+        if (r.host) e->pinned.put(r.host, r.hostBytes);
+        r.host = nullptr;
+        e->completed.erase(std::remove(e->completed.begin(), e->completed.end(), id), e->completed.end());
+        e->draining.erase(std::remove(e->draining.begin(), e->draining.end(), id), e->draining.end());
         e->reqs.erase(it);
         return KAM_OK;
     }

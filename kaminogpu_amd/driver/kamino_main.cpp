@@ -10,6 +10,8 @@
 //   seed=<u64> L=<int> density=band|noise geometric=0|1 format=bgeo|raw
 //   device=<int> particles=0|1 init=fbm|sh resume=<raw snapshot>
 //   resume_particles=<raw particles>
+// A run resumed from the raw snapshot of frame k (format=raw writes one per
+// frame) writes frames k+1 .. frames, equal to the uninterrupted run's.
 // The frame schedule is Kamino::run's (KaminoCore.cu:886-904): steps of dt
 // while T < i*DT, then one remainder step (which, as in the reference's
 // kernels, also advances by dt), then frame i is written to <gridPath><i>.bgeo
@@ -403,14 +405,29 @@ int main(int argc, char** argv) {
         for (size_t k = 0; k < 3 * n; ++k) color[k] = colorBGR[k + 1];
         return write_bgeo(pfile, (int)n, pos.data(), {{"v", 3, vel.data()}, {"color", 3, color.data()}});
     };
-    if (!write_frame(0)) {
+    // A resumed run continues the frame schedule of the run that wrote the
+    // snapshot: the snapshot's time is k DT after frame k, so the run goes on
+    // with frame k + 1 from T = k DT (the same float the uninterrupted run holds
+    // there) and does not rewrite frame k.
+    int first = 1;
+    float T = 0.0f;  // simulation time, float as in KaminoCore.cu:886
+    if (c.kv.count("resume")) {
+        const double kf = Tstart / (double)c.DT;
+        const long k = std::lround(kf);
+        if (k < 0 || k > c.frames || std::fabs(kf - (double)k) > 1e-3) {
+            std::cerr << "kamino: snapshot time " << Tstart << " is not a frame time of this config (frame "
+                      << kf << " of " << c.frames << ")\n";
+            return 2;
+        }
+        first = (int)k + 1;
+        T = (int)k * c.DT;
+    } else if (!write_frame(0)) {
         std::cerr << "kamino: cannot write frame 0 (" << kam_last_error() << ")\n";
         return 3;
     }
     const auto t0 = std::chrono::steady_clock::now();
     double stepMs = 0.0;
-    float T = 0.0f;  // simulation time, float as in KaminoCore.cu:886
-    for (int i = 1; i <= c.frames; ++i) {
+    for (int i = first; i <= c.frames; ++i) {
         int n = 0;
         while (T < i * c.DT) {
             ++n;
@@ -434,7 +451,8 @@ int main(int argc, char** argv) {
     }
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     std::cout << "Time spent: " << ms << "ms" << std::endl;
-    std::cout << "Performance: " << (ms > 0 ? 1000.0 * c.frames / ms : 0.0) << " frames per second" << std::endl;
+    std::cout << "Performance: " << (ms > 0 ? 1000.0 * (c.frames - first + 1) / ms : 0.0) << " frames per second"
+              << std::endl;
     std::cout << "Steps: " << step << ", stepping " << stepMs << " ms, "
               << (stepMs > 0 ? (double)step * nT * nP / (stepMs * 1e-3) : 0.0) << " cell-steps/s" << std::endl;
     kam_sim_destroy(sim);

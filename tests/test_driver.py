@@ -179,3 +179,74 @@ def _check_bgeo(b, n, attrs, radius=2.0):
     np.testing.assert_allclose(pts[:, 3], 1.0)
     assert b[-2:] == b"\x00\xff"
     return pts
+
+
+@pytest.mark.gpu
+def test_driver_bgeo_velocity_is_map_v_to_sphere(tmp_path):
+    """The grid .bgeo's v attribute of frame 0 against the closed form of
+    write_data_bgeo (KaminoSolver.cu:305-353) and mapVToSphere (:411-423),
+    evaluated here in numpy from the same seeded state: the cell-centred
+    (u_theta, u_phi) = ((v_N + v_S) / 2, (u_W + u_E) / 2), with
+    jNorth = jSouth = 0 in row 0 and nTheta - 2 in the last row, mapped to
+    (cos t cos p u_t - sin p u_p, -sin t u_t, cos t sin p u_t + cos p u_p) at
+    the cell centre (p, t) = (i h, (j + 1/2) h)."""
+    nt, R = 16, 2.0
+    write_cfg(tmp_path / "c.txt", [R, nt, 4, 0.01, 0.02, 0, 0, 1, 1, 1, 1,
+                                   str(tmp_path / "frame"), str(tmp_path / "part"), "null", "null",
+                                   "null", "init=sh", "seed=3", "L=4", "density=noise"])
+    r = run([str(tmp_path / "c.txt")], tmp_path)
+    assert r.returncode == 0, r.stderr
+    up, ut, rho = K.init_fields(nt, 4, 3, K.DENSITY_NOISE, radius=R)
+    up, ut = up.astype(np.float64), ut.astype(np.float64)
+    h = np.pi / nt
+    uph = 0.5 * (up + np.roll(up, -1, 1))
+    jn = np.r_[0, np.arange(nt - 2), nt - 2]
+    js = np.r_[0, np.arange(1, nt - 1), nt - 2]
+    uth = 0.5 * (ut[jn] + ut[js])
+    t = ((np.arange(nt) + 0.5) * h)[:, None]
+    p = (np.arange(2 * nt) * h)[None, :]
+    v = np.stack([np.cos(t) * np.cos(p) * uth - np.sin(p) * uph, -np.sin(t) * uth + 0 * p,
+                  np.cos(t) * np.sin(p) * uth + np.cos(p) * uph], -1).reshape(-1, 3)
+    pts = _check_bgeo(open(tmp_path / "frame0.bgeo", "rb").read(), nt * 2 * nt, [("v", 3), ("density", 1)],
+                      radius=R)
+    scale = np.abs(v).max()
+    assert scale > 1e-3
+    np.testing.assert_allclose(pts[:, 4:7], v, rtol=0, atol=2e-6 * scale)
+    np.testing.assert_array_equal(pts[:, 7], rho.reshape(-1))
+    # the pole rows use one u_theta row (jNorth = jSouth), not an average with a neighbour
+    row0 = pts[:2 * nt, 4:7]
+    np.testing.assert_allclose(row0[:, 1], -np.sin(h / 2) * ut[0], rtol=1e-5)
+
+
+@pytest.mark.gpu
+def test_driver_resume_continues_the_run(tmp_path):
+    """N frames straight equal k frames, then a run resumed from frame k's raw
+    snapshot (fields and particles) for the remaining N - k frames, bitwise;
+    the resumed run writes frames k+1 .. N and leaves frame k alone."""
+    nt, dt, DT, N, k = 32, 0.005, 0.0123, 4, 2
+    common = ["seed=5", "L=6", "density=noise", "format=raw", "init=sh"]
+
+    def cfg(name, frames, prefix):
+        write_cfg(tmp_path / name, [1.0, nt, 50, dt, DT, frames, 0, 1, 1, 1, 1,
+                                    str(tmp_path / (prefix + "f")), str(tmp_path / (prefix + "p")),
+                                    "null", "null", "null", *common])
+        return str(tmp_path / name)
+
+    r = run([cfg("straight.txt", N, "s")], tmp_path)
+    assert r.returncode == 0, r.stderr
+    r = run([cfg("first.txt", k, "a")], tmp_path)
+    assert r.returncode == 0, r.stderr
+    snap = (tmp_path / f"af{k}.kamr").read_bytes()
+    r = run([cfg("rest.txt", N, "a"), f"resume={tmp_path / f'af{k}.kamr'}",
+             f"resume_particles={tmp_path / f'ap{k}.kamp'}"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert f"Frame {k} is ready" not in r.stdout and f"Frame {N} is ready" in r.stdout
+    assert (tmp_path / f"af{k}.kamr").read_bytes() == snap
+    for i in range(k + 1, N + 1):
+        assert (tmp_path / f"af{i}.kamr").read_bytes() == (tmp_path / f"sf{i}.kamr").read_bytes()
+        assert (tmp_path / f"ap{i}.kamp").read_bytes() == (tmp_path / f"sp{i}.kamp").read_bytes()
+    # a snapshot that is not at a frame time of this config is refused
+    write_cfg(tmp_path / "bad2.txt", [1.0, nt, 50, dt, 0.0071, N, 0, 1, 1, 1, 1, str(tmp_path / "cf"),
+                                      str(tmp_path / "cp"), "null", "null", "null", *common])
+    r = run([str(tmp_path / "bad2.txt"), f"resume={tmp_path / f'af{k}.kamr'}"], tmp_path)
+    assert r.returncode == 2 and "frame time" in r.stderr
