@@ -44,7 +44,11 @@ enum {
 
 /* ---- precision modes ----------------------------------------------------- */
 enum {
-    KAM_PREC_FP32 = 0  /* FP32 fields, advection and FFT; FP64 tridiagonal solves */
+    KAM_PREC_FP32 = 0,          /* FP32 fields, advection and FFTs; FP64 tridiagonal solves */
+    KAM_PREC_FP64_PRESSURE = 1  /* as FP32, but the solved pressure spectrum is kept in FP64
+                                   and the inverse FFT and gradient subtraction run in FP64,
+                                   rounding each updated velocity once: the post-projection
+                                   divergence then sits at the FP32 storage floor */
 };
 
 /* Shape and parameters of one simulation.  Replaces the KaminoSolver ctor
@@ -102,7 +106,8 @@ int kam_sim_step_phases(kam_sim *sim, int phases);
 
 /* Copies the half spectrum buffer (float2 [M = nPhi/2][nTheta],
  * mode-major; slot 0 holds (mode 0, Nyquist)).  After KAM_PHASE_DIVFFT it
- * holds d-hat, after KAM_PHASE_TRI p-hat. */
+ * holds d-hat, after KAM_PHASE_TRI p-hat (with KAM_PREC_FP64_PRESSURE the
+ * solve stores p-hat in FP64 elsewhere and this buffer keeps d-hat). */
 int kam_sim_get_spectrum(kam_sim *sim, float *spec);
 
 /* Runs nsteps full steps and times them with HIP events on the simulation's

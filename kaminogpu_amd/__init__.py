@@ -23,6 +23,7 @@ PHASE_ADVECT, PHASE_DIVFFT, PHASE_TRI, PHASE_IFFT = 1, 8, 16, 32
 PHASE_PROJECT = 8 | 16 | 32
 PHASE_ALL = 1 | 8 | 16 | 32
 PREC_FP32 = 0
+PREC_FP64_PRESSURE = 1  # FP64 pressure spectrum, inverse FFT and gradient
 
 # every symbol include/kamino.h declares
 ABI_SYMBOLS = [

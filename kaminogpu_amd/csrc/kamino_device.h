@@ -10,6 +10,10 @@
 //                                           slot 0 packs (mode 0, Nyquist mode)
 // 28 bytes per cell in total (plus one 256-byte block for the watchdog word of
 // a stand-alone simulation; engine simulations keep theirs in a slot table).
+// With the FP64 pressure (SIM_P64), the solved spectrum p-hat is written as
+// double2 [M][nTheta] over the state copy the step has finished with (the
+// other parity: 2 of its 3 fields' worth of space), where the inverse
+// transform reads it; the spectrum buffer keeps d-hat.
 // Particles, when a simulation has them, live in their own allocation:
 //   part[b] float2 (phi, theta) [nPart], b = 0, 1 (ping-pong with the fields).  Per-size tables (row metrics, FP64 Poisson
 // rows, twiddles) are shared by every simulation of the same shape and dt.
@@ -36,10 +40,14 @@ enum RowTab {
 enum RowTabD {
     RD_COT2H = 0,    // cot(theta_j) / (2 h)
     RD_INVSIN2 = 1,  // 1 / sin^2(theta_j)
-    RD_COUNT = 2
+    RD_INVHS = 2,    // 1 / (h sin theta_j), the FP64 pressure gradient's phi factor
+    RD_COUNT = 3
 };
 
-enum SimFlags { SIM_GEOMETRIC = 1 };
+enum SimFlags {
+    SIM_GEOMETRIC = 1,
+    SIM_P64 = 2  // FP64 pressure: p-hat stored, transformed and differentiated in FP64
+};
 
 struct SimDesc {
     int nTheta, nPhi, M;   // M = nPhi / 2
@@ -52,6 +60,8 @@ struct SimDesc {
     const float* rowf;     // RT_COUNT * nTheta
     const double* rowd;    // RD_COUNT * nTheta
     const float* twN;      // float2 [nPhi]: exp(-2 pi i n / nPhi)
+    const double* twN64;   // double2 [nPhi]: the same in FP64 (the FP64 pressure transform)
+    double invH64;         // 1 / h in FP64
     int nfac;              // radix factors of M, innermost first
     int fac[12];
     int twStep[12];        // per pass p: twiddle index stride nPhi / (Ls r), Ls = fac[0] ... fac[p-1]

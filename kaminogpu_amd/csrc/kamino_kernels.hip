@@ -780,47 +780,59 @@ __global__ __launch_bounds__(256) void k_particles(const SimDesc* __restrict__ s
 // are separated by a wavefront-scope fence only, never a workgroup barrier.
 // DIR = -1 for exp(-i ...), +1 for exp(+i ...); unnormalised.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// Complex arithmetic on float2 (FP32 transforms) and double2 (the FP64
+// pressure transform); R is the component type.
+template <typename C> struct CplxT;
+template <> struct CplxT<float2> { typedef float R; };
+template <> struct CplxT<double2> { typedef double R; };
+__device__ __forceinline__ float2 mkc(float x, float y) { return make_float2(x, y); }
+__device__ __forceinline__ double2 mkc(double x, double y) { return make_double2(x, y); }
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+    return mkc(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) { return mkc(a.x + b.x, a.y + b.y); }
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) { return mkc(a.x - b.x, a.y - b.y); }
+template <typename C>
+__device__ __forceinline__ C cscale(C a, typename CplxT<C>::R s) { return mkc(a.x * s, a.y * s); }
 // DIR * i * a
-template <int DIR>
-__device__ __forceinline__ float2 cmuli(float2 a) {
-    return DIR > 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+template <int DIR, typename C>
+__device__ __forceinline__ C cmuli(C a) {
+    return DIR > 0 ? mkc(-a.y, a.x) : mkc(a.y, -a.x);
 }
 
-template <int RADIX, int DIR>
-__device__ __forceinline__ void butterfly(float2* x) {
+template <int RADIX, int DIR, typename C>
+__device__ __forceinline__ void butterfly(C* x) {
+    typedef typename CplxT<C>::R R;
     if constexpr (RADIX == 2) {
-        const float2 a = x[0], b = x[1];
+        const C a = x[0], b = x[1];
         x[0] = cadd(a, b);
         x[1] = csub(a, b);
     } else if constexpr (RADIX == 3) {
-        const float2 t = cadd(x[1], x[2]), u = csub(x[1], x[2]);
-        const float2 m = csub(x[0], cscale(t, 0.5f));
-        const float2 n = cmuli<DIR>(cscale(u, 0.86602540378443864676f));
+        const C t = cadd(x[1], x[2]), u = csub(x[1], x[2]);
+        const C m = csub(x[0], cscale(t, R(0.5)));
+        const C n = cmuli<DIR>(cscale(u, R(0.86602540378443864676)));
         x[0] = cadd(x[0], t);
         x[1] = cadd(m, n);
         x[2] = csub(m, n);
     } else if constexpr (RADIX == 4) {
-        const float2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
-        const float2 c = cadd(x[1], x[3]), d = cmuli<DIR>(csub(x[1], x[3]));
+        const C a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+        const C c = cadd(x[1], x[3]), d = cmuli<DIR>(csub(x[1], x[3]));
         x[0] = cadd(a, c);
         x[2] = csub(a, c);
         x[1] = cadd(b, d);
         x[3] = csub(b, d);
     } else {  // 5
-        const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
-        const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
-        const float2 t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]);
-        const float2 t3 = csub(x[1], x[4]), t4 = csub(x[2], x[3]);
-        const float2 a1 = cadd(x[0], cadd(cscale(t1, c1), cscale(t2, c2)));
-        const float2 a2 = cadd(x[0], cadd(cscale(t1, c2), cscale(t2, c1)));
-        const float2 b1 = cmuli<DIR>(cadd(cscale(t3, s1), cscale(t4, s2)));
-        const float2 b2 = cmuli<DIR>(csub(cscale(t3, s2), cscale(t4, s1)));
+        const R c1 = R(0.30901699437494742410), c2 = R(-0.80901699437494742410);
+        const R s1 = R(0.95105651629515357212), s2 = R(0.58778525229247312917);
+        const C t1 = cadd(x[1], x[4]), t2 = cadd(x[2], x[3]);
+        const C t3 = csub(x[1], x[4]), t4 = csub(x[2], x[3]);
+        const C a1 = cadd(x[0], cadd(cscale(t1, c1), cscale(t2, c2)));
+        const C a2 = cadd(x[0], cadd(cscale(t1, c2), cscale(t2, c1)));
+        const C b1 = cmuli<DIR>(cadd(cscale(t3, s1), cscale(t4, s2)));
+        const C b2 = cmuli<DIR>(csub(cscale(t3, s2), cscale(t4, s1)));
         x[0] = cadd(x[0], cadd(t1, t2));
         x[1] = cadd(a1, b1);
         x[4] = csub(a1, b1);
@@ -843,10 +855,10 @@ __host__ __device__ inline int fft_group(int M) {
     return g;
 }
 
-template <int RADIX, int DIR>
-__device__ __forceinline__ void wave_pass(const float2* __restrict__ src, float2* __restrict__ dst,
+template <int RADIX, int DIR, typename C>
+__device__ __forceinline__ void wave_pass(const C* __restrict__ src, C* __restrict__ dst,
                                           int M, int Ls, int lsShift, unsigned lsMagic,
-                                          const float2* __restrict__ tw, int twScale, int lane,
+                                          const C* __restrict__ tw, int twScale, int lane,
                                           int gs) {
     const int nb = M / RADIX;
     for (int b = lane; b < nb; b += gs) {
@@ -858,20 +870,20 @@ __device__ __forceinline__ void wave_pass(const float2* __restrict__ src, float2
             k = (int)__umulhi((unsigned)b, lsMagic);  // b / Ls, exact for b < 2^16
             j = b - k * Ls;
         }
-        const float2* in = src + j + k * Ls;
-        float2 x[RADIX];
+        const C* in = src + j + k * Ls;
+        C x[RADIX];
 #pragma unroll
         for (int q = 0; q < RADIX; ++q) {
-            float2 v = in[q * nb];
+            C v = in[q * nb];
             if (q > 0) {
-                float2 w = tw[q * j * twScale];
+                C w = tw[q * j * twScale];
                 if (DIR > 0) w.y = -w.y;
                 v = cmul(v, w);
             }
             x[q] = v;
         }
         butterfly<RADIX, DIR>(x);
-        float2* out = dst + j + k * Ls * RADIX;
+        C* out = dst + j + k * Ls * RADIX;
 #pragma unroll
         for (int m = 0; m < RADIX; ++m) out[m * Ls] = x[m];
     }
@@ -882,12 +894,12 @@ __device__ __forceinline__ void wave_pass(const float2* __restrict__ src, float2
 // exp(-2 pi i n / nPhi) staged in LDS, `lane` the lane within the group of gs
 // lanes.  Every lane of the wavefront calls it (rowless groups pass
 // active = false) so that the fences stay wavefront-uniform.
-template <int DIR>
-__device__ void wave_fft(float2* row, float2* scratch, const float2* tw, const SimDesc& s, int lane,
-                         int gs, bool active) {
+template <int DIR, typename C>
+__device__ void wave_fft(C* row, C* scratch, const C* tw, const SimDesc& s, int lane, int gs,
+                         bool active) {
     const int M = s.M;
-    float2* src = row;
-    float2* dst = scratch;
+    C* src = row;
+    C* dst = scratch;
     int Ls = 1;
     for (int p = 0; p < s.nfac; ++p) {
         const int r = s.fac[p];
@@ -903,7 +915,7 @@ __device__ void wave_fft(float2* row, float2* scratch, const float2* tw, const S
             }
         }
         wave_sync();
-        float2* tmp = src;
+        C* tmp = src;
         src = dst;
         dst = tmp;
         Ls *= r;
@@ -1066,12 +1078,16 @@ __device__ __forceinline__ double shfl_grp(double v, int srcLaneInGroup, int lan
 // global reads and writes stay contiguous, and lane l finds its row l K + k
 // at [k][l] (a padded row per k keeps the staging writes spread over the
 // banks).  RESID additionally writes, per mode, ||T x - d|| / ||d|| of the
-// FP64 solution x before it is rounded to FP32 (to SimDesc::resid).
+// FP64 solution x before it is rounded to FP32 (to SimDesc::resid).  A
+// simulation with the FP64 pressure (SIM_P64) keeps x in FP64: each lane
+// stores its rows straight to the double2 spectrum over the finished state
+// copy (parity ^ 1), K consecutive entries per lane, and skips the FP32
+// write-back.
 constexpr int kTriBatch = 4;  // float4 spectrum loads per thread in flight while staging
 
 template <int L, int KMAX, bool RESID>
 __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const Tile& tile,
-                                         float2* zsp, double* csp, double* ssp) {
+                                         int parity, float2* zsp, double* csp, double* ssp) {
     constexpr int G = 256 / L;  // modes per block
     (void)G;
     auto zs = [&](int gg, int k, int ll) -> float2& { return zsp[(gg * KMAX + k) * (L + 1) + ll]; };
@@ -1085,6 +1101,8 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
     {
         const SimDesc& s = sims[tile.sim];
         const int nT = s.nTheta;
+        const bool p64 = (s.flags & SIM_P64) != 0;
+        double2* __restrict__ spec64 = reinterpret_cast<double2*>(s.state[parity ^ 1][0]);
         const bool active = g < tile.rows;
         const int n = tile.row0 + (active ? g : 0);
         const int K = (nT + L - 1) / L;
@@ -1278,6 +1296,8 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                 if constexpr (RESID) {
                     xsr[k] = (n == 0) ? 0.0 : xr;
                     xsi[k] = xi;
+                } else if (p64) {
+                    spec64[(size_t)n * nT + r0 + k] = make_double2((n == 0) ? 0.0 : xr, xi);
                 } else {
                     zs(g, k, l) = make_float2((n == 0) ? 0.f : (float)xr, (float)xi);
                 }
@@ -1322,9 +1342,15 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
             if (active && l == 0) s.resid[n] = dd > 0.0 ? sqrt(rr / dd) : 0.0;
 #pragma unroll
             for (int k = 0; k < KMAX; ++k)
-                if (k < cnt) zs(g, k, l) = make_float2((float)xsr[k], (float)xsi[k]);
+                if (k < cnt) {
+                    if (p64)
+                        spec64[(size_t)n * nT + r0 + k] = make_double2(xsr[k], xsi[k]);
+                    else
+                        zs(g, k, l) = make_float2((float)xsr[k], (float)xsi[k]);
+                }
         }
         __syncthreads();
+        if (p64) return;  // the solution is already stored (uniform over the block)
         // write-back, pairs of entries as float4 where both lie in the tile
         float* __restrict__ blk = s.spec + b2;
         auto staged = [&](int e) -> float2 {
@@ -1348,10 +1374,11 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
 
 template <int L, int KMAX, bool RESID>
 __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sims,
-                                                 const Tile* __restrict__ tiles, int ntiles) {
+                                                 const Tile* __restrict__ tiles, int ntiles, int parity) {
     __shared__ float2 zs[(256 / L) * KMAX * (L + 1)];
     __shared__ double cs[KMAX * (L + 1)], ss[KMAX * (L + 1)];
-    for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) tri_tile<L, KMAX, RESID>(sims, tiles[t], zs, cs, ss);
+    for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles)
+        tri_tile<L, KMAX, RESID>(sims, tiles[t], parity, zs, cs, ss);
 }
 
 // Every class with K = 8 rows per lane in one launch: a tile carries its lanes
@@ -1359,17 +1386,18 @@ __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sim
 // would otherwise queue one small launch per class).
 template <bool RESID>
 __global__ __launch_bounds__(256) void k_tridiag_mixed(const SimDesc* __restrict__ sims,
-                                                       const Tile* __restrict__ tiles, int ntiles) {
+                                                       const Tile* __restrict__ tiles, int ntiles,
+                                                       int parity) {
     constexpr int K = 8, ZS = 32 * K * 9, CS = K * 65;  // largest over L = 8 ... 64
     __shared__ float2 zs[ZS];
     __shared__ double cs[CS], ss[CS];
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
         switch (tile.col0) {
-            case 8: tri_tile<8, K, RESID>(sims, tile, zs, cs, ss); break;
-            case 16: tri_tile<16, K, RESID>(sims, tile, zs, cs, ss); break;
-            case 32: tri_tile<32, K, RESID>(sims, tile, zs, cs, ss); break;
-            default: tri_tile<64, K, RESID>(sims, tile, zs, cs, ss); break;
+            case 8: tri_tile<8, K, RESID>(sims, tile, parity, zs, cs, ss); break;
+            case 16: tri_tile<16, K, RESID>(sims, tile, parity, zs, cs, ss); break;
+            case 32: tri_tile<32, K, RESID>(sims, tile, parity, zs, cs, ss); break;
+            default: tri_tile<64, K, RESID>(sims, tile, parity, zs, cs, ss); break;
         }
     }
 }
@@ -1382,54 +1410,83 @@ __global__ __launch_bounds__(256) void k_tridiag_mixed(const SimDesc* __restrict
 // (cuFFT CUFFT_FORWARD plus shiftUKernel, KaminoCore.cu:682-703, with the
 // n = 0 mode dropped), then the gradient subtraction of applyPressureTheta
 // (:705-722) and applyPressurePhi (:724-747), in place.
+//
+// C = float2 is the FP32 form (p-hat from the spectrum buffer, FP32 twiddles
+// and gradient factors); C = double2 the FP64 pressure (SIM_P64): p-hat from
+// the double2 spectrum over the finished state copy, FP64 twiddles, FFT and
+// differences, one rounding of each updated velocity to FP32.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_spec(Rsrc r, int idx, float2& v) { v = ldf2(r, 2 * idx); }
+__device__ __forceinline__ void load_spec(Rsrc r, int idx, double2& v) {
+    v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16, 0, 0));
+}
+
+// Stages the nPhi FP64 twiddles of s into LDS, one double2 (16 bytes) each.
+__device__ __forceinline__ void stage_twiddles(double2* twl, const SimDesc& s, int tid, int nthr) {
+    const Rsrc tr = make_rsrc(reinterpret_cast<const float*>(s.twN64), 4 * s.nPhi);
+    float4* dst = reinterpret_cast<float4*>(twl);
+    for (int i0 = tid; i0 < s.nPhi; i0 += 4 * nthr) {
+        float4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = ldf4(tr, 4 * (i0 + q * nthr));
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (i0 + q * nthr < s.nPhi) dst[i0 + q * nthr] = v[q];
+    }
+}
+
+template <typename C>
 __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restrict__ sims,
                                                    const Tile* __restrict__ tiles, int ntiles,
                                                    int parity) {
-    extern __shared__ __attribute__((aligned(16))) float2 zlds[];
+    typedef typename CplxT<C>::R R;
+    constexpr bool kP64 = std::is_same<C, double2>::value;
+    extern __shared__ __attribute__((aligned(16))) float4 zlds4[];
+    C* zlds = reinterpret_cast<C*>(zlds4);
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
         const SimDesc& s = sims[tile.sim];
         const int nT = s.nTheta, nP = s.nPhi, M = s.M;
-        const int j0 = tile.row0, R = tile.rows;
-        const int Rp = min(R + 1, nT - j0);  // rows of p needed
+        const int j0 = tile.row0, R_ = tile.rows;
+        const int Rp = min(R_ + 1, nT - j0);  // rows of p needed
         const int gsh = s.fftShift, gs = 1 << gsh;  // lanes per row (a power of two)
         // kFftBlock threads: kFftBlock / gs lane groups (a power of two)
         const int ngsh = kFftBlockShift - gsh, grp = tid >> gsh, ngrp = 1 << ngsh, lane = tid & (gs - 1);
-        float2* tw = zlds;                                          // nPhi twiddles
-        float2* zbuf = zlds + nP;                                   // Rp rows of M
-        float2* scratch = zbuf + (size_t)(Rp + grp) * M;            // this lane group's row
-        const Rsrc specR = make_rsrc(s.spec, 2 * M * nT);
+        C* tw = zlds;                                               // nPhi twiddles
+        C* zbuf = zlds + nP;                                        // Rp rows of M
+        C* scratch = zbuf + (size_t)(Rp + grp) * M;                 // this lane group's row
+        const Rsrc specR = kP64 ? make_rsrc(s.state[parity ^ 1][0], 4 * M * nT)
+                                : make_rsrc(s.spec, 2 * M * nT);
         stage_twiddles(tw, s, tid, nthr);
         __syncthreads();
         // ceil(2^32 / Rp) (Rp = 1 splits nothing); idx / Rp is exact for idx < 2^16
         const unsigned magicR = (Rp == 1) ? 0u : 0xffffffffu / (unsigned)Rp + 1u;
         for (int idx0 = tid; idx0 < Rp * M; idx0 += kSpecBatch * nthr) {
-            float2 Pn[kSpecBatch], Pm[kSpecBatch];
+            C Pn[kSpecBatch], Pm[kSpecBatch];
 #pragma unroll
             for (int q = 0; q < kSpecBatch; ++q) {
                 const int idx = idx0 + q * nthr;
                 const int n = (Rp == 1) ? idx : (int)__umulhi((unsigned)idx, magicR), j = j0 + idx - n * Rp;
                 // past the spectrum (idx beyond the tile, or M - n = M) reads as 0
-                Pn[q] = ldf2(specR, 2 * (n * nT + j));
-                Pm[q] = ldf2(specR, 2 * ((M - n) * nT + j));
+                load_spec(specR, n * nT + j, Pn[q]);
+                load_spec(specR, (M - n) * nT + j, Pm[q]);
             }
 #pragma unroll
             for (int q = 0; q < kSpecBatch; ++q) {
                 const int idx = idx0 + q * nthr;
                 if (idx >= Rp * M) break;
                 const int n = (Rp == 1) ? idx : (int)__umulhi((unsigned)idx, magicR), r = idx - n * Rp;
-                float2 Y;
+                C Y;
                 if (n == 0) {
-                    Y = make_float2(Pn[q].y, -Pn[q].y);  // P_0 = 0 (mode dropped), P_M = Nyquist
+                    Y = mkc(Pn[q].y, -Pn[q].y);  // P_0 = 0 (mode dropped), P_M = Nyquist
                 } else {
-                    const float2 ev = make_float2(Pn[q].x + Pm[q].x, Pn[q].y - Pm[q].y);
-                    const float2 od = make_float2(Pn[q].x - Pm[q].x, Pn[q].y + Pm[q].y);
-                    float2 w = tw[n];
+                    const C ev = mkc(Pn[q].x + Pm[q].x, Pn[q].y - Pm[q].y);
+                    const C od = mkc(Pn[q].x - Pm[q].x, Pn[q].y + Pm[q].y);
+                    C w = tw[n];
                     w.y = -w.y;  // exp(+i pi n / M)
-                    const float2 o = cmul(w, od);
-                    Y = make_float2(ev.x - o.y, ev.y + o.x);
+                    const C o = cmul(w, od);
+                    Y = mkc(ev.x - o.y, ev.y + o.x);
                 }
                 zbuf[(size_t)r * M + n] = Y;
             }
@@ -1438,22 +1495,25 @@ __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restri
         const int nPass = (Rp + ngrp - 1) >> ngsh;
         for (int it = 0; it < nPass; ++it) {
             const int r = it * ngrp + grp;
-            wave_fft<+1>(zbuf + (size_t)(r < Rp ? r : 0) * M, scratch, tw, s, lane, gs, r < Rp);
+            wave_fft<+1>(zbuf + (size_t)(r < Rp ? r : 0) * M, scratch, (const C*)tw, s, lane, gs, r < Rp);
         }
         __syncthreads();
-        const float* p = reinterpret_cast<const float*>(zbuf);  // p[r][i], row stride nP
+        const R* p = reinterpret_cast<const R*>(zbuf);  // p[r][i], row stride nP
         float* __restrict__ up = s.state[parity][F_UPHI];
         float* __restrict__ ut = s.state[parity][F_UTHETA];
         const Rsrc upR = make_rsrc(up, nT * nP), utR = make_rsrc(ut, (nT - 1) * nP);
-        const Rsrc hsR = make_rsrc(s.rowf + RT_INVHS * nT, nT);
-        const float invH = s.invH;
+        const Rsrc hsR = kP64 ? make_rsrc(reinterpret_cast<const float*>(s.rowd + RD_INVHS * nT), 2 * nT)
+                              : make_rsrc(s.rowf + RT_INVHS * nT, nT);
+        R invH;
+        if constexpr (kP64) invH = s.invH64;
+        else invH = s.invH;
         // column pairs (i, i + 1) (nPhi is even); a batch's u loads all go out
         // before its first store, and u_theta of the last row reads as 0
-        const int half = nP >> 1, nPair = R * half;
+        const int half = nP >> 1, nPair = R_ * half;
         const unsigned magicH = s.magicP2;  // idx / (nPhi / 2)
         for (int idx0 = tid; idx0 < nPair; idx0 += kGradBatch * nthr) {
             float2 u[kGradBatch], v[kGradBatch];
-            float hs[kGradBatch];
+            R hs[kGradBatch];
 #pragma unroll
             for (int q = 0; q < kGradBatch; ++q) {
                 const int idx = idx0 + q * nthr;
@@ -1461,7 +1521,8 @@ __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restri
                 const int o = (j0 + r) * nP + i;
                 u[q] = ldf2(upR, o);
                 v[q] = ldf2(utR, o);
-                hs[q] = ldf(hsR, j0 + r);
+                if constexpr (kP64) hs[q] = ldd(hsR, j0 + r);
+                else hs[q] = ldf(hsR, j0 + r);
             }
 #pragma unroll
             for (int q = 0; q < kGradBatch; ++q) {
@@ -1469,15 +1530,16 @@ __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restri
                 if (idx >= nPair) break;
                 const int r = (int)__umulhi((unsigned)idx, magicH), i = 2 * (idx - r * half), j = j0 + r;
                 const int il = (i == 0) ? nP - 1 : i - 1;
-                const float* pr = p + (size_t)r * nP;
-                const float p0 = pr[i], p1 = pr[i + 1];
-                const float2 nu = make_float2(u[q].x - (p0 - pr[il]) * hs[q], u[q].y - (p1 - p0) * hs[q]);
+                const R* pr = p + (size_t)r * nP;
+                const R p0 = pr[i], p1 = pr[i + 1];
+                const float2 nu = make_float2((float)((R)u[q].x - (p0 - pr[il]) * hs[q]),
+                                              (float)((R)u[q].y - (p1 - p0) * hs[q]));
                 const size_t o = (size_t)j * nP + i;
                 *reinterpret_cast<float2*>(up + o) = nu;
                 bool bad = !isfinite(nu.x) || !isfinite(nu.y);
                 if (j < nT - 1) {
-                    const float2 nv = make_float2(v[q].x - (pr[nP + i] - p0) * invH,
-                                                  v[q].y - (pr[nP + i + 1] - p1) * invH);
+                    const float2 nv = make_float2((float)((R)v[q].x - (pr[nP + i] - p0) * invH),
+                                                  (float)((R)v[q].y - (pr[nP + i + 1] - p1) * invH));
                     *reinterpret_cast<float2*>(ut + o) = nv;
                     bad |= !isfinite(nv.x) || !isfinite(nv.y);
                 }
@@ -1528,22 +1590,24 @@ hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, in
 }
 
 hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles,
-                          int ntiles, int gridCap, hipStream_t st, bool resid) {
+                          int ntiles, int parity, int gridCap, hipStream_t st, bool resid) {
     if (ntiles <= 0) return hipSuccess;
     const dim3 grid(grid_for(ntiles, gridCap)), block(256);
     if (lanesPerMode == 0 && kmax == 8) {
         if (resid)
-            hipLaunchKernelGGL((k_tridiag_mixed<true>), grid, block, 0, st, sims, tiles, ntiles);
+            hipLaunchKernelGGL((k_tridiag_mixed<true>), grid, block, 0, st, sims, tiles, ntiles, parity);
         else
-            hipLaunchKernelGGL((k_tridiag_mixed<false>), grid, block, 0, st, sims, tiles, ntiles);
+            hipLaunchKernelGGL((k_tridiag_mixed<false>), grid, block, 0, st, sims, tiles, ntiles, parity);
         return hipGetLastError();
     }
 #define KAM_TRI(LL, KK)                                                                        \
     if (lanesPerMode == LL && kmax == KK) {                                                    \
         if (resid)                                                                             \
-            hipLaunchKernelGGL((k_tridiag<LL, KK, true>), grid, block, 0, st, sims, tiles, ntiles); \
+            hipLaunchKernelGGL((k_tridiag<LL, KK, true>), grid, block, 0, st, sims, tiles, ntiles, \
+                               parity);                                                        \
         else                                                                                   \
-            hipLaunchKernelGGL((k_tridiag<LL, KK, false>), grid, block, 0, st, sims, tiles, ntiles); \
+            hipLaunchKernelGGL((k_tridiag<LL, KK, false>), grid, block, 0, st, sims, tiles, ntiles, \
+                               parity);                                                        \
         return hipGetLastError();                                                              \
     }
     KAM_TRI(8, 8)
@@ -1556,10 +1620,14 @@ hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const
 }
 
 hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
-                            size_t ldsBytes, int gridCap, hipStream_t st) {
+                            size_t ldsBytes, int gridCap, hipStream_t st, bool p64) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ifft_grad, dim3(grid_for(ntiles, gridCap)), dim3(kFftBlock), ldsBytes, st, sims,
-                       tiles, ntiles, parity);
+    if (p64)
+        hipLaunchKernelGGL(k_ifft_grad<double2>, dim3(grid_for(ntiles, gridCap)), dim3(kFftBlock), ldsBytes,
+                           st, sims, tiles, ntiles, parity);
+    else
+        hipLaunchKernelGGL(k_ifft_grad<float2>, dim3(grid_for(ntiles, gridCap)), dim3(kFftBlock), ldsBytes,
+                           st, sims, tiles, ntiles, parity);
     return hipGetLastError();
 }
 
@@ -1578,7 +1646,8 @@ static hipError_t allow_lds(const void* fn, size_t bytes) {
 
 hipError_t configure_kernels(size_t bytes) {
     const void* fns[] = {(const void*)k_advect<adv_stride(64)>, (const void*)k_advect<adv_stride(128)>,
-                         (const void*)k_div_fft, (const void*)k_ifft_grad};
+                         (const void*)k_div_fft, (const void*)k_ifft_grad<float2>,
+                         (const void*)k_ifft_grad<double2>};
     for (const void* fn : fns) {
         const hipError_t e = allow_lds(fn, bytes);
         if (e != hipSuccess) return e;

@@ -19,10 +19,13 @@ hipError_t launch_particles(const SimDesc* sims, const Tile* tiles, int ntiles, 
                             int gridCap, hipStream_t st);
 hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
                           size_t ldsBytes, int gridCap, hipStream_t st);
+// `parity` names the current state copy: a simulation with the FP64 pressure
+// keeps its solved spectrum over the other one.
 hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles,
-                          int ntiles, int gridCap, hipStream_t st, bool resid = false);
+                          int ntiles, int parity, int gridCap, hipStream_t st, bool resid = false);
+// p64: the tiles belong to simulations with the FP64 pressure (SIM_P64).
 hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
-                            size_t ldsBytes, int gridCap, hipStream_t st);
+                            size_t ldsBytes, int gridCap, hipStream_t st, bool p64 = false);
 
 // Raises the dynamic-LDS limit of every step kernel to `bytes` (call once).
 hipError_t configure_kernels(size_t bytes);

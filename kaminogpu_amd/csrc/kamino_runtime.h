@@ -49,6 +49,7 @@ struct Tables {
     float* rowf = nullptr;
     double* rowd = nullptr;
     float* twN = nullptr;
+    double* twN64 = nullptr;
 };
 int acquire_tables(int device, int nTheta, double dt, double radius, Tables* out);
 void release_tables(int device, int nTheta, double dt, double radius);
@@ -76,8 +77,9 @@ public:
     // returns the parity after the step (flipped iff the advection ran).
     int launch(int phases, int parity, hipStream_t st, int* newParity,
                hipEvent_t* kernelEvents = nullptr);
-    // Queues the tridiagonal phase alone, with the per-mode residual report.
-    int launch_tri_residual(hipStream_t st);
+    // Queues the tridiagonal phase alone, with the per-mode residual report
+    // (`parity`: the current state copy).
+    int launch_tri_residual(hipStream_t st, int parity);
     int nsims() const { return nsims_; }
     size_t capacityBytes() const { return capSims_ * sizeof(SimDesc) + capTiles_ * sizeof(Tile); }
 
@@ -93,10 +95,11 @@ private:
     std::vector<Tile> hTiles_;
     int nsims_ = 0;
     int advOff_ = 0, nAdv_ = 0, fftOff_ = 0, nFft_ = 0, partOff_ = 0, nPart_ = 0;
+    int nFft32_ = 0, nFft64_ = 0;  // FFT tiles of FP32- and FP64-pressure simulations
     std::vector<TriClass> tri_;
-    size_t advLds_ = 0, fftLds_ = 0;
+    size_t advLds_ = 0, fftLds_ = 0, fftLds64_ = 0;
     int advStride_ = 0;  // staged row stride of the advection launch
-    int launch_tri(hipStream_t st, bool resid);
+    int launch_tri(hipStream_t st, bool resid, int parity);
 };
 
 // Kernel-time categories reported by profiling.

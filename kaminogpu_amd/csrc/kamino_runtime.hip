@@ -71,10 +71,12 @@ static int fft_group_host(int M) {
     return g;
 }
 // LDS of an FFT tile: the nPhi = 2M twiddles, R + 1 rows of M complex and the
-// lane groups' scratch rows
-static size_t fft_lds(int R, int M) {
-    return (size_t)(2 + R + 1 + 256 / fft_group_host(M)) * M * 2 * sizeof(float);
+// lane groups' scratch rows; FP32 complex (8 bytes), or FP64 (16 bytes) for the
+// inverse transform of the FP64 pressure
+static size_t fft_lds(int R, int M, bool p64 = false) {
+    return (size_t)(2 + R + 1 + 256 / fft_group_host(M)) * M * (p64 ? 16 : 8);
 }
+static constexpr size_t kLdsMax = 160 * 1024;
 
 // ---------------------------------------------------------------------------
 // Tuning table: tile shapes per n_theta, set by the autotuner (or by hand).
@@ -188,6 +190,7 @@ int acquire_tables(int device, int nT, double dt, double radius, Tables* out) {
     std::vector<float> rowf((size_t)RT_COUNT * nT);
     std::vector<double> rowd((size_t)RD_COUNT * nT);
     std::vector<float> tw((size_t)2 * nP);
+    std::vector<double> tw64((size_t)2 * nP);
     for (int j = 0; j < nT; ++j) {
         const double tc = (j + 0.5) * h, tt = (j + 1.0) * h;
         const double sc = std::sin(tc), cc = std::cos(tc);
@@ -199,20 +202,25 @@ int acquire_tables(int device, int nT, double dt, double radius, Tables* out) {
         rowf[RT_INVHS * nT + j] = (float)(1.0 / (h * sc));
         rowd[RD_COT2H * nT + j] = cc / (2.0 * h * sc);
         rowd[RD_INVSIN2 * nT + j] = 1.0 / (sc * sc);
+        rowd[RD_INVHS * nT + j] = 1.0 / (h * sc);
     }
     for (int n = 0; n < nP; ++n) {
         const double a = 2.0 * kPi * n / nP;
-        tw[2 * n] = (float)std::cos(a);
-        tw[2 * n + 1] = (float)(-std::sin(a));
+        tw64[2 * n] = std::cos(a);
+        tw64[2 * n + 1] = -std::sin(a);
+        tw[2 * n] = (float)tw64[2 * n];
+        tw[2 * n + 1] = (float)tw64[2 * n + 1];
     }
     KAM_HIP(hipSetDevice(device));
     TabEntry e;
     KAM_HIP(hipMalloc(&e.t.rowf, rowf.size() * sizeof(float)));
     KAM_HIP(hipMalloc(&e.t.rowd, rowd.size() * sizeof(double)));
     KAM_HIP(hipMalloc(&e.t.twN, tw.size() * sizeof(float)));
+    KAM_HIP(hipMalloc(&e.t.twN64, tw64.size() * sizeof(double)));
     KAM_HIP(hipMemcpy(e.t.rowf, rowf.data(), rowf.size() * sizeof(float), hipMemcpyHostToDevice));
     KAM_HIP(hipMemcpy(e.t.rowd, rowd.data(), rowd.size() * sizeof(double), hipMemcpyHostToDevice));
     KAM_HIP(hipMemcpy(e.t.twN, tw.data(), tw.size() * sizeof(float), hipMemcpyHostToDevice));
+    KAM_HIP(hipMemcpy(e.t.twN64, tw64.data(), tw64.size() * sizeof(double), hipMemcpyHostToDevice));
     e.refs = 1;
     g_tabs[key] = e;
     *out = e.t;
@@ -228,6 +236,7 @@ void release_tables(int device, int nT, double dt, double radius) {
     (void)hipFree(it->second.t.rowf);
     (void)hipFree(it->second.t.rowd);
     (void)hipFree(it->second.t.twN);
+    (void)hipFree(it->second.t.twN64);
     g_tabs.erase(it);
 }
 
@@ -237,10 +246,11 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
     d.nTheta = p.nT;
     d.nPhi = p.nP;
     d.M = p.M;
-    d.flags = sd.geometric ? SIM_GEOMETRIC : 0;
+    d.flags = (sd.geometric ? SIM_GEOMETRIC : 0) | (sd.precision == KAM_PREC_FP64_PRESSURE ? SIM_P64 : 0);
     d.cofTheta = (float)(sd.dt / (sd.radius * h));
     d.invH = (float)(1.0 / h);
     d.invH2 = 1.0 / (h * h);
+    d.invH64 = 1.0 / h;
     const size_t N = (size_t)p.nT * p.nP;
     for (int b = 0; b < 2; ++b)
         for (int f = 0; f < 3; ++f) d.state[b][f] = mem + (size_t)(3 * b + f) * N;
@@ -249,6 +259,7 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
     d.rowf = t.rowf;
     d.rowd = t.rowd;
     d.twN = t.twN;
+    d.twN64 = t.twN64;
     d.nfac = p.nfac;
     int Ls = 1;
     for (int k = 0; k < p.nfac; ++k) {
@@ -339,7 +350,7 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
                        hipStream_t st) {
     hTiles_.clear();
     tri_.clear();
-    advLds_ = fftLds_ = 0;
+    advLds_ = fftLds_ = fftLds64_ = 0;
     const int minTiles = 2 * device_cus();
     const size_t n = plans.size();
     std::vector<int> advRows(n), advCols(n), fftRows(n);
@@ -393,14 +404,28 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
         advLds_ = std::max(advLds_, adv_lds(advRows[s], W, advStride_, (sims[s].flags & SIM_GEOMETRIC) != 0));
     }
     nAdv_ = (int)hTiles_.size();
+    // FFT tiles: simulations with the FP32 pressure first, then those with the
+    // FP64 pressure, whose inverse transform is a launch of its own (FP64 rows
+    // in LDS: their tiles hold as many rows as fit)
     fftOff_ = (int)hTiles_.size();
-    for (size_t s = 0; s < sims.size(); ++s) {
-        const Plan& p = plans[s];
-        for (int j = 0; j < p.nT; j += fftRows[s])
-            hTiles_.push_back(Tile{(int)s, j, std::min(fftRows[s], p.nT - j), 0});
-        fftLds_ = std::max(fftLds_, fft_lds(fftRows[s], p.M));
-    }
+    nFft64_ = 0;
+    for (int pass = 0; pass < 2; ++pass)
+        for (size_t s = 0; s < sims.size(); ++s) {
+            const Plan& p = plans[s];
+            const bool p64 = (sims[s].flags & SIM_P64) != 0;
+            if (p64 != (pass == 1)) continue;
+            int R = fftRows[s];
+            if (p64)
+                while (R > 1 && fft_lds(R, p.M, true) > kLdsMax) --R;
+            for (int j = 0; j < p.nT; j += R) {
+                hTiles_.push_back(Tile{(int)s, j, std::min(R, p.nT - j), 0});
+                nFft64_ += p64 ? 1 : 0;
+            }
+            fftLds_ = std::max(fftLds_, fft_lds(R, p.M));
+            if (p64) fftLds64_ = std::max(fftLds64_, fft_lds(R, p.M, true));
+        }
     nFft_ = (int)hTiles_.size() - fftOff_;
+    nFft32_ = nFft_ - nFft64_;
     // tridiagonal tiles, grouped by kernel class; when several classes of
     // K = 8 rows per lane are present they share one mixed launch (L = 0),
     // each tile carrying its lanes per mode in col0
@@ -458,17 +483,17 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
 
 static const int kGridCap = 1 << 20;
 
-int DeviceBatch::launch_tri(hipStream_t st, bool resid) {
+int DeviceBatch::launch_tri(hipStream_t st, bool resid, int parity) {
     // one launch per class (all K = 8 classes of a mixed batch form one, L = 0),
     // in stream order (side streams would let small classes overlap, but their
     // fork/join graphs cost far more to capture each time a serving engine's
     // membership changes)
     for (const TriClass& c : tri_)
-        KAM_HIP(launch_tridiag(c.L, c.K, dSims_, dTiles_ + c.off, c.count, kGridCap, st, resid));
+        KAM_HIP(launch_tridiag(c.L, c.K, dSims_, dTiles_ + c.off, c.count, parity, kGridCap, st, resid));
     return KAM_OK;
 }
 
-int DeviceBatch::launch_tri_residual(hipStream_t st) { return launch_tri(st, true); }
+int DeviceBatch::launch_tri_residual(hipStream_t st, int parity) { return launch_tri(st, true, parity); }
 
 int DeviceBatch::launch(int phases, int parity, hipStream_t st, int* newParity,
                         hipEvent_t* ev) {
@@ -491,13 +516,15 @@ int DeviceBatch::launch(int phases, int parity, hipStream_t st, int* newParity,
     }
     if (phases & KAM_PHASE_TRI) {
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_TRI], st));
-        int rc = launch_tri(st, false);
+        int rc = launch_tri(st, false, p);
         if (rc != KAM_OK) return rc;
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_TRI + 1], st));
     }
     if (phases & KAM_PHASE_IFFT) {
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_IFFT], st));
-        KAM_HIP(launch_ifft_grad(dSims_, dTiles_ + fftOff_, nFft_, p, fftLds_, kGridCap, st));
+        KAM_HIP(launch_ifft_grad(dSims_, dTiles_ + fftOff_, nFft32_, p, fftLds_, kGridCap, st, false));
+        KAM_HIP(launch_ifft_grad(dSims_, dTiles_ + fftOff_ + nFft32_, nFft64_, p, fftLds64_, kGridCap, st,
+                                 true));
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_IFFT + 1], st));
     }
     *newParity = p;
@@ -583,7 +610,8 @@ static int check_desc(const kam_sim_desc* d, Plan& plan) {
     if (!d) return set_err(KAM_EINVAL, "null descriptor");
     if (!(d->dt > 0.0) || !std::isfinite(d->dt)) return set_err(KAM_EINVAL, "dt must be > 0");
     if (!(d->radius > 0.0) || !std::isfinite(d->radius)) return set_err(KAM_EINVAL, "radius must be > 0");
-    if (d->precision != KAM_PREC_FP32) return set_err(KAM_EINVAL, "unknown precision %d", d->precision);
+    if (d->precision != KAM_PREC_FP32 && d->precision != KAM_PREC_FP64_PRESSURE)
+        return set_err(KAM_EINVAL, "unknown precision %d", d->precision);
     return make_plan(d->n_theta, d->n_phi, plan);
 }
 
@@ -732,7 +760,7 @@ int kam_sim_poisson_residual(kam_sim* s, double* per_mode, double* max_rel, doub
         int rc = s->batch.build(sims, plans, s->stream);
         if (rc != KAM_OK) return rc;
     }
-    int rc = s->batch.launch_tri_residual(s->stream);
+    int rc = s->batch.launch_tri_residual(s->stream, s->parity);
     if (rc != KAM_OK) return rc;
     std::vector<double> h((size_t)M);
     KAM_HIP(hipMemcpyAsync(h.data(), s->resid, (size_t)M * sizeof(double), hipMemcpyDeviceToHost, s->stream));

@@ -90,12 +90,13 @@ class Engine:
     def __exit__(self, *a):
         self.close()
 
-    def admit(self, n_theta, dt, u_phi, u_theta, rho, steps, radius=1.0, geometric=True):
-        """Returns the request id, or None when the engine has no room now."""
-        from . import PREC_FP32, SimDesc, _f32, _ptr
+    def admit(self, n_theta, dt, u_phi, u_theta, rho, steps, radius=1.0, geometric=True, precision=0):
+        """Returns the request id, or None when the engine has no room now.
+        precision: PREC_FP32 or PREC_FP64_PRESSURE."""
+        from . import SimDesc, _f32, _ptr
 
         nt, nph = int(n_theta), 2 * int(n_theta)
-        sd = SimDesc(nt, nph, float(radius), float(dt), int(bool(geometric)), PREC_FP32, self.device)
+        sd = SimDesc(nt, nph, float(radius), float(dt), int(bool(geometric)), int(precision), self.device)
         a, b = _f32(u_phi, (nt, nph)), _f32(u_theta, (nt - 1, nph))
         c = None if rho is None else _f32(rho, (nt, nph))
         rid = C.c_uint64(0)

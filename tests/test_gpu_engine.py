@@ -14,8 +14,8 @@ pytestmark = pytest.mark.gpu
 SIZES = [32, 48, 64, 96, 128, 160, 192, 256]
 
 
-def alone(nt, dt, state, steps, geometric=True):
-    with K.Sim(nt, dt, geometric=geometric) as s:
+def alone(nt, dt, state, steps, geometric=True, precision=K.PREC_FP32):
+    with K.Sim(nt, dt, geometric=geometric, precision=precision) as s:
         s.set_fields(*state)
         s.step(steps)
         return s.get_fields()
@@ -39,6 +39,24 @@ def test_uneven_batch_equals_alone_bitwise():
             for g, x in zip(got, ref):
                 np.testing.assert_array_equal(g, x)
     log(f"[engine] uneven batch of {len(reqs)} sims bitwise equal to alone")
+
+
+def test_mixed_precision_batch_equals_alone_bitwise():
+    """FP32- and FP64-pressure simulations of several sizes in one batch (the
+    inverse transform goes out as one launch per precision): each equals its
+    solo run bitwise."""
+    sizes = [48, 64, 75, 128, 256, 375]
+    reqs = [make_request(nt, 500 + k) for k, nt in enumerate(sizes)]
+    precs = [K.PREC_FP64_PRESSURE if k % 2 else K.PREC_FP32 for k in range(len(sizes))]
+    with K.Engine() as eng:
+        ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=5, precision=p) for r, p in zip(reqs, precs)]
+        assert all(i is not None for i in ids)
+        eng.step(5)
+        got = [eng.fetch(i, r["nt"]) for r, i in zip(reqs, ids)]
+    for r, p, g in zip(reqs, precs, got):
+        for a, b in zip(g, alone(r["nt"], r["dt"], r["state"], 5, precision=p)):
+            np.testing.assert_array_equal(a, b)
+    log(f"[engine] batch of FP32- and FP64-pressure sims {sizes} bitwise equal to alone")
 
 
 def test_mixed_tile_widths_equal_alone_bitwise():
