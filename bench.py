@@ -50,11 +50,43 @@ def parse():
                     help="tuning cache: sizes it holds are loaded, the others autotuned and appended")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this size (for HBM counter scaling)")
+    ap.add_argument("--clock-warmup-s", type=float, default=CLOCK_WARMUP_S,
+                    help="untimed steps for this many seconds before each leg's warm-up steps "
+                         "(the GPU's clocks ramp over the first ~50 ms of work)")
     return ap.parse_args()
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+CLOCK_WARMUP_S = 0.3  # --clock-warmup-s
+
+
+def clock_warmup_engine(eng, seconds):
+    """Runs untimed engine steps for `seconds` of wall time before a leg's own
+    warm-up steps: the GPU raises its clocks over the first ~50 ms of steady
+    work (measured on the 128-sphere batch: 5 warm-up steps give 0.39-0.40 ms
+    per step, 200 give 0.331), so a short warm-up would time the ramp, not the
+    step.  Returns the number of steps run."""
+    import time
+
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        eng.step(10)
+        eng.sync()
+        n += 10
+    return n
+
+
+def clock_warmup_sim(sim, seconds):
+    import time
+
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        sim.benchmark(50)
+        n += 50
+    return n
 
 
 def init_state(nt, L, seed, dens):
@@ -116,7 +148,8 @@ def run_uneven(K, steps, cache, sims=64, seed=3):
     with K.Engine(use_graph=USE_GRAPH) as eng:
         for k, nt in enumerate(sizes):
             st = K.init_fields(int(nt), min(16, int(nt) // 4), 100 + k, K.DENSITY_BAND)
-            eng.admit(int(nt), 0.5 / nt, *st, steps=steps + 20)
+            eng.admit(int(nt), 0.5 / nt, *st, steps=10 ** 6)
+        clock_warmup_engine(eng, CLOCK_WARMUP_S)
         eng.step(10)
         eng.reset_stats()
         eng.step(steps)
@@ -164,6 +197,7 @@ def run_large(K, steps, cache, nt=1024):
     log(f"large tiles: {tiles}")
     with K.Sim(nt, dt) as s:
         s.set_fields(*init_state(nt, 32, 2, 1))
+        clock_warmup_sim(s, CLOCK_WARMUP_S)
         s.benchmark(10)
         total, per, _ = s.benchmark(steps)
         _, _, kern = s.benchmark(10, per_kernel=True)
@@ -236,8 +270,9 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
 
 def main():
     a = parse()
-    global USE_GRAPH
+    global USE_GRAPH, CLOCK_WARMUP_S
     USE_GRAPH = a.launch == "graph"
+    CLOCK_WARMUP_S = a.clock_warmup_s
     if a.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has one MI355X; "
               "multi-GPU sharding is design-only.", file=sys.stderr)
@@ -258,6 +293,7 @@ def main():
         state = init_state(nt, 16, 1, 1)
         with K.Sim(nt, dt) as s:
             s.set_fields(*state)
+            clock_warmup_sim(s, CLOCK_WARMUP_S)
             s.benchmark(20)
             p_total, p_steps, _ = s.benchmark(a.plain_steps)
         plain = cells * a.plain_steps / (p_total * 1e-3)
@@ -269,20 +305,33 @@ def main():
 
     # ---- batched form: B spheres through the engine (graph-replayed or directly launched steps)
     states = [init_state(nt, 16, 1 + b, 1) for b in range(min(B, 8))]
-    total_steps = a.warmup + a.steps + a.profile_steps + 10
-    with K.Engine(use_graph=USE_GRAPH) as eng:
-        for b in range(B):
-            if eng.admit(nt, dt, *states[b % len(states)], steps=total_steps) is None:
-                raise SystemExit("engine refused the batch")
-        eng.step(a.warmup)
-        eng.reset_stats()
-        eng.step(a.steps)
-        st = eng.stats()
-        kern_ms = eng.profile(a.profile_steps)
+
+    def batched_leg(precision):
+        with K.Engine(use_graph=USE_GRAPH) as eng:
+            for b in range(B):
+                if eng.admit(nt, dt, *states[b % len(states)], steps=10 ** 6, precision=precision) is None:
+                    raise SystemExit("engine refused the batch")
+            clock_steps = clock_warmup_engine(eng, CLOCK_WARMUP_S)
+            eng.step(a.warmup)
+            eng.reset_stats()
+            eng.step(a.steps)
+            st = eng.stats()
+            kern_ms = eng.profile(a.profile_steps)
+        return st, kern_ms, clock_steps
+
+    st, kern_ms, clock_steps = batched_leg(K.PREC_FP32)
     ms_per_step = st.gpu_ms / st.steps
     batched = st.cell_steps / (st.gpu_ms * 1e-3)
     log(f"batched x{B}: {batched:.3e} cell-steps/s, {ms_per_step:.4f} ms/step, "
         f"p50 {st.step_p50_ms:.4f} p99 {st.step_p99_ms:.4f}")
+    # the same batch with the FP64 pressure (post-projection divergence at the
+    # FP32 storage floor)
+    st64, kern64, _ = batched_leg(K.PREC_FP64_PRESSURE)
+    p64 = {"cell_steps_per_s": st64.cell_steps / (st64.gpu_ms * 1e-3), "ms_per_step": st64.gpu_ms / st64.steps,
+           "p50_step_ms": st64.step_p50_ms, "p99_step_ms": st64.step_p99_ms,
+           "kernel_ms": dict(zip(KERNELS, [float(x) for x in kern64]))}
+    log(f"batched x{B} FP64 pressure: {p64['cell_steps_per_s']:.3e} cell-steps/s, "
+        f"{p64['ms_per_step']:.4f} ms/step, kernel ms {np.round(kern64, 4)}")
 
     large = run_large(K, a.large_steps, a.tune_cache) if a.large_steps > 0 else None
     uneven = run_uneven(K, a.uneven_steps, a.tune_cache) if a.uneven_steps > 0 else None
@@ -313,9 +362,10 @@ def main():
         "data": "synthetic (seeded spherical-harmonic streamfunction, density band)",
         "config": {"workload": f"batched: {B} spheres of {nt}x{2 * nt}, dt {dt}, one engine step "
                                f"advances all {B}", "n_theta": nt, "n_phi": 2 * nt, "sims": B,
-                   "tiles": tiles, "launch": a.launch},
+                   "tiles": tiles, "launch": a.launch, "clock_warmup_steps": clock_steps},
         "batched": {"cell_steps_per_s": batched, "p50_step_ms": st.step_p50_ms,
                     "p99_step_ms": st.step_p99_ms, "speedup_vs_plain": batched / plain if plain else None},
+        "batched_fp64_pressure": p64,
         "plain": {"cell_steps_per_s": plain, "steps": a.plain_steps,
                   "p50_step_ms": float(np.percentile(p_steps, 50)),
                   "p99_step_ms": float(np.percentile(p_steps, 99))},

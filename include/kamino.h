@@ -56,7 +56,8 @@ enum {
  * caller had to make (KaminoCore.cu:868-872): nPhi = 2 nTheta, h = pi/nTheta
  * (KaminoCore.cu:849). */
 typedef struct kam_sim_desc {
-    int n_theta;     /* latitude rows, >= 4; nTheta must be 2^a 3^b 5^c */
+    int n_theta;     /* latitude rows, 4 ... 2048, of the form 2^a 3^b 5^c (1024 at most
+                        with KAM_PREC_FP64_PRESSURE) */
     int n_phi;       /* must equal 2 * n_theta */
     double radius;   /* sphere radius R */
     double dt;       /* time step (constant, as timeStepGlobal, KaminoCore.cu:871) */
@@ -109,6 +110,16 @@ int kam_sim_step_phases(kam_sim *sim, int phases);
  * holds d-hat, after KAM_PHASE_TRI p-hat (with KAM_PREC_FP64_PRESSURE the
  * solve stores p-hat in FP64 elsewhere and this buffer keeps d-hat). */
 int kam_sim_get_spectrum(kam_sim *sim, float *spec);
+
+/* Pressure of the last projection (host, float p[nTheta][nPhi], the
+ * reference's pressure quantity, KaminoSolver.cu:54, as shiftUKernel leaves
+ * it, KaminoCore.cu:682-703: the n = 0 mode removed): the inverse transform
+ * of the solved spectrum, recomputed on demand.  Valid after a full step or
+ * KAM_PHASE_TRI, until the next step's advection (FP64 pressure) or
+ * KAM_PHASE_DIVFFT (FP32 pressure).  The reference keeps p on the device
+ * and never writes it out; the first call allocates a device buffer of 4
+ * bytes per cell. */
+int kam_sim_get_pressure(kam_sim *sim, float *p);
 
 /* Runs nsteps full steps and times them with HIP events on the simulation's
  * stream.  step_ms[nsteps] (optional) receives each step's duration,

@@ -37,6 +37,7 @@ ABI_SYMBOLS = [
     "kam_engine_create", "kam_engine_destroy", "kam_engine_admit", "kam_engine_step",
     "kam_engine_poll", "kam_engine_fetch", "kam_engine_cancel", "kam_engine_sync",
     "kam_engine_stats", "kam_engine_reset_stats", "kam_engine_profile", "kam_engine_now",
+    "kam_sim_get_pressure",
 ]
 
 
@@ -75,6 +76,7 @@ def lib():
     L.kam_sim_step.argtypes = [C.c_void_p, C.c_int]
     L.kam_sim_step_phases.argtypes = [C.c_void_p, C.c_int]
     L.kam_sim_get_spectrum.argtypes = [C.c_void_p, _fp]
+    L.kam_sim_get_pressure.argtypes = [C.c_void_p, _fp]
     L.kam_sim_sync.argtypes = [C.c_void_p]
     L.kam_sim_status.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     _dpp = C.POINTER(C.c_double)
@@ -265,6 +267,12 @@ class Sim:
         s = np.empty((M, self.n_theta, 2), np.float32)
         check(lib().kam_sim_get_spectrum(self._h, _ptr(s)))
         return s[..., 0] + 1j * s[..., 1]
+
+    def pressure(self):
+        """Pressure of the last projection, float32 [nTheta][nPhi] (n = 0 mode removed)."""
+        p = np.empty((self.n_theta, self.n_phi), np.float32)
+        check(lib().kam_sim_get_pressure(self._h, _ptr(p)))
+        return p
 
     def set_particles(self, coords):
         """Particles as (n, 2) (phi, theta) radians; later steps move them."""

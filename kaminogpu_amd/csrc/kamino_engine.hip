@@ -463,12 +463,12 @@ int kam_engine_admit(kam_engine* e, const kam_sim_desc* d, const float* u_phi,
     if (!e || !d || !u_phi || !u_theta || !id) return set_err(KAM_EINVAL, "null argument");
     if (steps <= 0) return set_err(KAM_EINVAL, "steps must be > 0");
     if (d->device != e->ed.device) return set_err(KAM_EINVAL, "simulation device differs from the engine's");
-    if (!(d->dt > 0.0) || !(d->radius > 0.0) ||
-        (d->precision != KAM_PREC_FP32 && d->precision != KAM_PREC_FP64_PRESSURE))
-        return set_err(KAM_EINVAL, "bad dt / radius / precision");
+    if (!(d->dt > 0.0) || !(d->radius > 0.0)) return set_err(KAM_EINVAL, "bad dt / radius");
     KAM_HIP(hipSetDevice(e->ed.device));
     Request r;
     int rc = make_plan(d->n_theta, d->n_phi, r.plan);
+    if (rc != KAM_OK) return rc;
+    rc = check_precision(d->precision, r.plan);
     if (rc != KAM_OK) return rc;
     if ((int)e->active.size() >= e->ed.max_slots) {
         e->rejected++;

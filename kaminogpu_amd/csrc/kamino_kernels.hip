@@ -214,19 +214,30 @@ __device__ __forceinline__ float geo_root(float G, float vb, float ub) {
     if (!(fabsf(G) > 1e-7f)) return ub;
     const float b = fmaf(G, vb, 1.f);
     if (b > 0.f) {
-        // the root is bounded by both |ub| / b and cbrt(|ub| / G^2); Newton on
-        // the monotone, convex-on-each-side cubic from the smaller bound
-        // converges from above, quadratically within a few steps
         const float a = G * G;
-        float x = ub * __builtin_amdgcn_rcpf(b);
-        if (a * x * x > b) x = copysignf(cbrtf(fabsf(ub) * __builtin_amdgcn_rcpf(a)), ub);
+        const float rb = __builtin_amdgcn_rcpf(b);
+        const float y = ub * rb;
+        // x = y phi(e) with e = a y^2 / b >= 0, where e phi^3 + phi - 1 = 0:
+        // phi = 1 - e + 3 e^2 - 12 e^3 + 55 e^4 - ...  For e <= 3e-3 the
+        // truncation error 55 e^4 |y| stays below 5e-9 |x|, under FP32
+        // resolution; this covers every row but the few nearest the poles
+        const float e = a * y * y * rb;
+        bool done = e <= 3e-3f;
+        float x = y * fmaf(e, fmaf(e, fmaf(e, -12.f, 3.f), -1.f), 1.f);
+        if (__all(done)) return x;
+        // the others: the root is bounded by both |ub| / b and cbrt(|ub| / G^2);
+        // Newton on the monotone, convex-on-each-side cubic from the smaller
+        // bound converges from above, quadratically within a few steps
+        if (!done) {
+            x = y;
+            if (a * x * x > b) x = copysignf(cbrtf(fabsf(ub) * __builtin_amdgcn_rcpf(a)), ub);
+        }
         // at most five steps.  Converging from above, the error after a step
         // of dx is at most 3 (dx / x)^2 |x| (f'' / 2f' <= 3 a |x| / (3 a x^2 + b)
         // between the iterate and the root), so a lane stops once |dx| <= 1e-4 |x|
         // (error below 3e-8 |x|, under FP32 resolution) and then no longer
-        // updates (its result never depends on its neighbours); the wavefront
-        // leaves when every lane has (most cells need one step)
-        bool done = false;
+        // updates; a lane's result depends on its own inputs only, never on
+        // its neighbours (the wavefront leaves when every lane has stopped)
         for (int k = 0; k < 5; ++k) {
             const float x2 = x * x;
             const float f = fmaf(a * x2, x, fmaf(b, x, -ub));
@@ -1414,7 +1425,9 @@ __global__ __launch_bounds__(256) void k_tridiag_mixed(const SimDesc* __restrict
 // C = float2 is the FP32 form (p-hat from the spectrum buffer, FP32 twiddles
 // and gradient factors); C = double2 the FP64 pressure (SIM_P64): p-hat from
 // the double2 spectrum over the finished state copy, FP64 twiddles, FFT and
-// differences, one rounding of each updated velocity to FP32.
+// differences, one rounding of each updated velocity to FP32.  PONLY writes
+// the tile's pressure rows to pout (float [nTheta][nPhi], one simulation)
+// instead of updating the velocities: the pressure readback.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void load_spec(Rsrc r, int idx, float2& v) { v = ldf2(r, 2 * idx); }
 __device__ __forceinline__ void load_spec(Rsrc r, int idx, double2& v) {
@@ -1435,10 +1448,10 @@ __device__ __forceinline__ void stage_twiddles(double2* twl, const SimDesc& s, i
     }
 }
 
-template <typename C>
+template <typename C, bool PONLY>
 __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restrict__ sims,
                                                    const Tile* __restrict__ tiles, int ntiles,
-                                                   int parity) {
+                                                   int parity, float* __restrict__ pout) {
     typedef typename CplxT<C>::R R;
     constexpr bool kP64 = std::is_same<C, double2>::value;
     extern __shared__ __attribute__((aligned(16))) float4 zlds4[];
@@ -1499,6 +1512,11 @@ __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restri
         }
         __syncthreads();
         const R* p = reinterpret_cast<const R*>(zbuf);  // p[r][i], row stride nP
+        if constexpr (PONLY) {
+            for (int idx = tid; idx < R_ * nP; idx += nthr) pout[(size_t)j0 * nP + idx] = (float)p[idx];
+            __syncthreads();
+            continue;
+        }
         float* __restrict__ up = s.state[parity][F_UPHI];
         float* __restrict__ ut = s.state[parity][F_UTHETA];
         const Rsrc upR = make_rsrc(up, nT * nP), utR = make_rsrc(ut, (nT - 1) * nP);
@@ -1615,19 +1633,27 @@ hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const
     KAM_TRI(32, 8)
     KAM_TRI(64, 8)
     KAM_TRI(64, 16)
+    KAM_TRI(64, 32)
 #undef KAM_TRI
     return hipErrorInvalidValue;
 }
 
 hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
-                            size_t ldsBytes, int gridCap, hipStream_t st, bool p64) {
+                            size_t ldsBytes, int gridCap, hipStream_t st, bool p64, float* pout) {
     if (ntiles <= 0) return hipSuccess;
-    if (p64)
-        hipLaunchKernelGGL(k_ifft_grad<double2>, dim3(grid_for(ntiles, gridCap)), dim3(kFftBlock), ldsBytes,
-                           st, sims, tiles, ntiles, parity);
+    const dim3 grid(grid_for(ntiles, gridCap)), block(kFftBlock);
+    if (p64 && pout)
+        hipLaunchKernelGGL((k_ifft_grad<double2, true>), grid, block, ldsBytes, st, sims, tiles, ntiles, parity,
+                           pout);
+    else if (p64)
+        hipLaunchKernelGGL((k_ifft_grad<double2, false>), grid, block, ldsBytes, st, sims, tiles, ntiles, parity,
+                           pout);
+    else if (pout)
+        hipLaunchKernelGGL((k_ifft_grad<float2, true>), grid, block, ldsBytes, st, sims, tiles, ntiles, parity,
+                           pout);
     else
-        hipLaunchKernelGGL(k_ifft_grad<float2>, dim3(grid_for(ntiles, gridCap)), dim3(kFftBlock), ldsBytes,
-                           st, sims, tiles, ntiles, parity);
+        hipLaunchKernelGGL((k_ifft_grad<float2, false>), grid, block, ldsBytes, st, sims, tiles, ntiles, parity,
+                           pout);
     return hipGetLastError();
 }
 
@@ -1646,8 +1672,9 @@ static hipError_t allow_lds(const void* fn, size_t bytes) {
 
 hipError_t configure_kernels(size_t bytes) {
     const void* fns[] = {(const void*)k_advect<adv_stride(64)>, (const void*)k_advect<adv_stride(128)>,
-                         (const void*)k_div_fft, (const void*)k_ifft_grad<float2>,
-                         (const void*)k_ifft_grad<double2>};
+                         (const void*)k_div_fft, (const void*)k_ifft_grad<float2, false>,
+                         (const void*)k_ifft_grad<double2, false>, (const void*)k_ifft_grad<float2, true>,
+                         (const void*)k_ifft_grad<double2, true>};
     for (const void* fn : fns) {
         const hipError_t e = allow_lds(fn, bytes);
         if (e != hipSuccess) return e;

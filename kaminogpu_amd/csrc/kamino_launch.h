@@ -24,8 +24,11 @@ hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, in
 hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles,
                           int ntiles, int parity, int gridCap, hipStream_t st, bool resid = false);
 // p64: the tiles belong to simulations with the FP64 pressure (SIM_P64).
+// pout (one simulation only): write its pressure rows there instead of
+// updating the velocities.
 hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
-                            size_t ldsBytes, int gridCap, hipStream_t st, bool p64 = false);
+                            size_t ldsBytes, int gridCap, hipStream_t st, bool p64 = false,
+                            float* pout = nullptr);
 
 // Raises the dynamic-LDS limit of every step kernel to `bytes` (call once).
 hipError_t configure_kernels(size_t bytes);

@@ -39,6 +39,8 @@ struct Plan {
 
 // Validates a shape and fills its plan; returns KAM_OK or KAM_EINVAL.
 int make_plan(int nTheta, int nPhi, Plan& plan);
+// KAM_OK when `precision` is known and supported at this shape.
+int check_precision(int precision, const Plan& plan);
 // Overrides the tile heights (0 keeps the default); used by the autotuner.
 int adjust_plan(Plan& plan, int advRows, int fftRows);
 // Records tile shapes for n_theta (0 = default); later plans use them.
@@ -54,6 +56,11 @@ struct Tables {
 int acquire_tables(int device, int nTheta, double dt, double radius, Tables* out);
 void release_tables(int device, int nTheta, double dt, double radius);
 
+// Host values of the per-shape tables (no allocation): rowf[RT_COUNT * nT],
+// rowd[RD_COUNT * nT], tw[2 nPhi] (FP32) and tw64[2 nPhi] (FP64).
+void compute_tables(int nTheta, double dt, double radius, float* rowf, double* rowd, float* tw,
+                    double* tw64);
+
 // Fills the host descriptor of a simulation whose state lives at `mem`
 // (7 * nTheta * nPhi floats: two state copies of three fields, then the
 // spectrum).
@@ -66,6 +73,32 @@ size_t sim_floats(int nTheta, int nPhi);
 struct TriClass {
     int L, K, off, count;
 };
+
+// Tile lists of one step over a set of simulations, as ranges of one array.
+struct TileLayout {
+    int nsims = 0, total = 0;
+    int advOff = 0, nAdv = 0;
+    int fftOff = 0, nFft = 0, nFft32 = 0, nFft64 = 0;  // FFT tiles (FP32-pressure ones first)
+    int partOff = 0, nPart = 0;
+    int ntri = 0;
+    TriClass tri[6] = {};
+    size_t advLds = 0, fftLds = 0, fftLds64 = 0;
+    int advStride = 0;  // staged row stride of the advection launch
+};
+
+// Plans the tiles of n simulations: writes the first min(cap, needed) tiles to
+// `out` and returns how many the plan needs (uses no heap).  rowScratch holds
+// 3n ints.  sims may be given narrower advection tiles (set_tile_cols) when
+// the batch mixes tile widths.
+int plan_tiles(SimDesc* sims, const Plan* plans, int n, int* rowScratch, Tile* out, int cap,
+               TileLayout& layout);
+// Queues the phases of one step over device descriptors and tiles planned as
+// `layout`; returns the parity after the step (flipped iff the advection ran).
+int launch_step(const SimDesc* dSims, const Tile* dTiles, const TileLayout& layout, int phases,
+                int parity, hipStream_t st, int* newParity, hipEvent_t* kernelEvents = nullptr);
+// Queues the tridiagonal phase alone (resid: with the per-mode residual report).
+int launch_tri(const SimDesc* dSims, const Tile* dTiles, const TileLayout& layout, int parity,
+               hipStream_t st, bool resid);
 
 class DeviceBatch {
 public:
@@ -80,7 +113,10 @@ public:
     // Queues the tridiagonal phase alone, with the per-mode residual report
     // (`parity`: the current state copy).
     int launch_tri_residual(hipStream_t st, int parity);
-    int nsims() const { return nsims_; }
+    // Queues the inverse transform of the solved spectrum of a one-simulation
+    // batch, writing its pressure to pout (device, float [nTheta][nPhi]).
+    int launch_pressure(hipStream_t st, int parity, float* pout);
+    int nsims() const { return lay_.nsims; }
     size_t capacityBytes() const { return capSims_ * sizeof(SimDesc) + capTiles_ * sizeof(Tile); }
 
 private:
@@ -93,13 +129,7 @@ private:
     size_t stageBytes_ = 0;
     hipEvent_t uploaded_ = nullptr;
     std::vector<Tile> hTiles_;
-    int nsims_ = 0;
-    int advOff_ = 0, nAdv_ = 0, fftOff_ = 0, nFft_ = 0, partOff_ = 0, nPart_ = 0;
-    int nFft32_ = 0, nFft64_ = 0;  // FFT tiles of FP32- and FP64-pressure simulations
-    std::vector<TriClass> tri_;
-    size_t advLds_ = 0, fftLds_ = 0, fftLds64_ = 0;
-    int advStride_ = 0;  // staged row stride of the advection launch
-    int launch_tri(hipStream_t st, bool resid, int parity);
+    TileLayout lay_;
 };
 
 // Kernel-time categories reported by profiling.

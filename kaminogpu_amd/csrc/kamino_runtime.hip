@@ -103,7 +103,7 @@ static TuneEntry tune_get(int nT) {
 int make_plan(int nT, int nP, Plan& p) {
     if (nT < 4) return set_err(KAM_EINVAL, "n_theta must be >= 4 (got %d)", nT);
     if (nP != 2 * nT) return set_err(KAM_EINVAL, "n_phi must equal 2*n_theta (got %d, %d)", nP, nT);
-    if (nT > 1024) return set_err(KAM_EINVAL, "n_theta must be <= 1024 (got %d)", nT);
+    if (nT > 2048) return set_err(KAM_EINVAL, "n_theta must be <= 2048 (got %d)", nT);
     p = Plan();
     p.nT = nT;
     p.nP = nP;
@@ -131,7 +131,8 @@ int make_plan(int nT, int nP, Plan& p) {
     else if (nT <= 128) { p.triL = 16; p.triK = 8; }
     else if (nT <= 256) { p.triL = 32; p.triK = 8; }
     else if (nT <= 512) { p.triL = 64; p.triK = 8; }
-    else { p.triL = 64; p.triK = 16; }
+    else if (nT <= 1024) { p.triL = 64; p.triK = 16; }
+    else { p.triL = 64; p.triK = 32; }
     // tuned shapes, then manual overrides for tile sweeps
     const TuneEntry te = tune_get(nT);
     if (te.advCols >= 8 && te.advCols <= kAdvMaxCols) p.advCols = te.advCols < nP ? te.advCols : nP;
@@ -164,6 +165,14 @@ int adjust_plan(Plan& p, int advRows, int fftRows) {
     return KAM_OK;
 }
 
+int check_precision(int precision, const Plan& plan) {
+    if (precision != KAM_PREC_FP32 && precision != KAM_PREC_FP64_PRESSURE)
+        return set_err(KAM_EINVAL, "unknown precision %d", precision);
+    if (precision == KAM_PREC_FP64_PRESSURE && fft_lds(1, plan.M, true) > kLdsMax)
+        return set_err(KAM_EINVAL, "the FP64 pressure needs n_theta <= 1024 (its FFT rows must fit in LDS)");
+    return KAM_OK;
+}
+
 size_t sim_floats(int nT, int nP) { return (size_t)7 * nT * nP + 64; }
 
 // ---------------------------------------------------------------------------
@@ -176,21 +185,9 @@ struct TabEntry {
 static std::mutex g_tab_mu;
 static std::map<std::tuple<int, int, double, double>, TabEntry> g_tabs;
 
-int acquire_tables(int device, int nT, double dt, double radius, Tables* out) {
-    std::lock_guard<std::mutex> lk(g_tab_mu);
-    auto key = std::make_tuple(device, nT, dt, radius);
-    auto it = g_tabs.find(key);
-    if (it != g_tabs.end()) {
-        it->second.refs++;
-        *out = it->second.t;
-        return KAM_OK;
-    }
+void compute_tables(int nT, double dt, double radius, float* rowf, double* rowd, float* tw, double* tw64) {
     const int nP = 2 * nT;
     const double h = kPi / nT;  // KaminoCore.cu:849
-    std::vector<float> rowf((size_t)RT_COUNT * nT);
-    std::vector<double> rowd((size_t)RD_COUNT * nT);
-    std::vector<float> tw((size_t)2 * nP);
-    std::vector<double> tw64((size_t)2 * nP);
     for (int j = 0; j < nT; ++j) {
         const double tc = (j + 0.5) * h, tt = (j + 1.0) * h;
         const double sc = std::sin(tc), cc = std::cos(tc);
@@ -211,6 +208,23 @@ int acquire_tables(int device, int nT, double dt, double radius, Tables* out) {
         tw[2 * n] = (float)tw64[2 * n];
         tw[2 * n + 1] = (float)tw64[2 * n + 1];
     }
+}
+
+int acquire_tables(int device, int nT, double dt, double radius, Tables* out) {
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto key = std::make_tuple(device, nT, dt, radius);
+    auto it = g_tabs.find(key);
+    if (it != g_tabs.end()) {
+        it->second.refs++;
+        *out = it->second.t;
+        return KAM_OK;
+    }
+    const int nP = 2 * nT;
+    std::vector<float> rowf((size_t)RT_COUNT * nT);
+    std::vector<double> rowd((size_t)RD_COUNT * nT);
+    std::vector<float> tw((size_t)2 * nP);
+    std::vector<double> tw64((size_t)2 * nP);
+    compute_tables(nT, dt, radius, rowf.data(), rowd.data(), tw.data(), tw64.data());
     KAM_HIP(hipSetDevice(device));
     TabEntry e;
     KAM_HIP(hipMalloc(&e.t.rowf, rowf.size() * sizeof(float)));
@@ -331,30 +345,34 @@ int DeviceBatch::reserve(size_t ns, size_t nt, hipStream_t st) {
 // Tile heights: each simulation's planned height, halved for every
 // simulation while a kernel would get fewer than two workgroups per CU.  The
 // height changes only how rows are grouped, never a value.
-static int shrink_until_busy(const std::vector<Plan>& plans, std::vector<int>& rows, int minTiles,
-                             const int* advCols) {
+static int shrink_until_busy(const Plan* plans, int n, int* rows, int minTiles, const int* advCols) {
     for (;;) {
         long tiles = 0;
         bool canShrink = false;
-        for (size_t s = 0; s < plans.size(); ++s) {
+        for (int s = 0; s < n; ++s) {
             const long cols = advCols ? (plans[s].nP + advCols[s] - 1) / advCols[s] : 1;
             tiles += cols * ((plans[s].nT + rows[s] - 1) / rows[s]);
             canShrink |= rows[s] > 1;
         }
         if (tiles >= minTiles || !canShrink) return (int)tiles;
-        for (int& r : rows) r = r > 1 ? (r + 1) / 2 : 1;
+        for (int s = 0; s < n; ++s) rows[s] = rows[s] > 1 ? (rows[s] + 1) / 2 : 1;
     }
 }
 
-int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Plan>& plans,
-                       hipStream_t st) {
-    hTiles_.clear();
-    tri_.clear();
-    advLds_ = fftLds_ = fftLds64_ = 0;
+int plan_tiles(SimDesc* sims, const Plan* plans, int n, int* rowScratch, Tile* out, int cap,
+               TileLayout& L) {
+    L = TileLayout();
+    L.nsims = n;
+    int cnt = 0;
+    auto put = [&](const Tile& t) {
+        if (cnt < cap) out[cnt] = t;
+        ++cnt;
+    };
     const int minTiles = 2 * device_cus();
-    const size_t n = plans.size();
-    std::vector<int> advRows(n), advCols(n), fftRows(n);
-    for (size_t s = 0; s < n; ++s) {
+    int* advRows = rowScratch;
+    int* advCols = rowScratch + n;
+    int* fftRows = rowScratch + 2 * n;
+    for (int s = 0; s < n; ++s) {
         advRows[s] = plans[s].advRows;
         advCols[s] = plans[s].advCols;
         fftRows[s] = plans[s].fftRows;
@@ -364,28 +382,27 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
     // different widths, the wide ones (alone a launch too small to fill the
     // GPU) are restaged as narrow tiles of the same area, so that the narrow
     // majority keeps its occupancy.  Tile shapes never change a value.
-    std::vector<SimDesc> sims(simsIn);
     bool narrow = false, wide = false;
-    for (size_t s = 0; s < n; ++s) (advCols[s] > kAdvMaxCols / 2 ? wide : narrow) = true;
+    for (int s = 0; s < n; ++s) (advCols[s] > kAdvMaxCols / 2 ? wide : narrow) = true;
     if (narrow && wide)
-        for (size_t s = 0; s < n; ++s)
+        for (int s = 0; s < n; ++s)
             if (advCols[s] > kAdvMaxCols / 2) {
                 advRows[s] = std::min(2 * advRows[s], kAdvMaxRows);
                 advCols[s] = kAdvMaxCols / 2;
                 set_tile_cols(sims[s], advCols[s]);
             }
-    advStride_ = adv_stride(wide && !narrow ? kAdvMaxCols : kAdvMaxCols / 2);
-    shrink_until_busy(plans, advRows, minTiles, advCols.data());
-    shrink_until_busy(plans, fftRows, minTiles, nullptr);
+    L.advStride = adv_stride(wide && !narrow ? kAdvMaxCols : kAdvMaxCols / 2);
+    shrink_until_busy(plans, n, advRows, minTiles, advCols);
+    shrink_until_busy(plans, n, fftRows, minTiles, nullptr);
     // advection tiles, in batch order (tallest simulations first); the last
     // column tile of a row band overlaps its neighbour instead of running short.
     // A tile starting within kAdvHaloRows + 1 rows of a pole (or ending within
     // them) samples with the checked sampler throughout, so the two polar bands
     // are exactly that tall when the tile height allows, and every band between
     // them can take the fast one; the last inner band may be shorter.
-    advOff_ = 0;
+    L.advOff = 0;
     constexpr int kPoleRows = kAdvHaloRows + 1;
-    for (size_t s = 0; s < n; ++s) {
+    for (int s = 0; s < n; ++s) {
         const Plan& p = plans[s];
         const int W = advCols[s], R = advRows[s];
         const bool polar = R >= kPoleRows && p.nT > 2 * kPoleRows;
@@ -398,19 +415,17 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
                 rows = p.nT - j;
             else
                 rows = std::min(R, p.nT - kPoleRows - j);
-            for (int c = 0; c < p.nP; c += W)
-                hTiles_.push_back(Tile{(int)s, j, rows, std::min(c, p.nP - W)});
+            for (int c = 0; c < p.nP; c += W) put(Tile{s, j, rows, std::min(c, p.nP - W)});
         }
-        advLds_ = std::max(advLds_, adv_lds(advRows[s], W, advStride_, (sims[s].flags & SIM_GEOMETRIC) != 0));
+        L.advLds = std::max(L.advLds, adv_lds(advRows[s], W, L.advStride, (sims[s].flags & SIM_GEOMETRIC) != 0));
     }
-    nAdv_ = (int)hTiles_.size();
+    L.nAdv = cnt;
     // FFT tiles: simulations with the FP32 pressure first, then those with the
     // FP64 pressure, whose inverse transform is a launch of its own (FP64 rows
     // in LDS: their tiles hold as many rows as fit)
-    fftOff_ = (int)hTiles_.size();
-    nFft64_ = 0;
+    L.fftOff = cnt;
     for (int pass = 0; pass < 2; ++pass)
-        for (size_t s = 0; s < sims.size(); ++s) {
+        for (int s = 0; s < n; ++s) {
             const Plan& p = plans[s];
             const bool p64 = (sims[s].flags & SIM_P64) != 0;
             if (p64 != (pass == 1)) continue;
@@ -418,51 +433,112 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
             if (p64)
                 while (R > 1 && fft_lds(R, p.M, true) > kLdsMax) --R;
             for (int j = 0; j < p.nT; j += R) {
-                hTiles_.push_back(Tile{(int)s, j, std::min(R, p.nT - j), 0});
-                nFft64_ += p64 ? 1 : 0;
+                put(Tile{s, j, std::min(R, p.nT - j), 0});
+                L.nFft64 += p64 ? 1 : 0;
             }
-            fftLds_ = std::max(fftLds_, fft_lds(R, p.M));
-            if (p64) fftLds64_ = std::max(fftLds64_, fft_lds(R, p.M, true));
+            L.fftLds = std::max(L.fftLds, fft_lds(R, p.M));
+            if (p64) L.fftLds64 = std::max(L.fftLds64, fft_lds(R, p.M, true));
         }
-    nFft_ = (int)hTiles_.size() - fftOff_;
-    nFft32_ = nFft_ - nFft64_;
+    L.nFft = cnt - L.fftOff;
+    L.nFft32 = L.nFft - L.nFft64;
     // tridiagonal tiles, grouped by kernel class; when several classes of
     // K = 8 rows per lane are present they share one mixed launch (L = 0),
     // each tile carrying its lanes per mode in col0
-    const int classes[5][2] = {{8, 8}, {16, 8}, {32, 8}, {64, 8}, {64, 16}};
+    const int classes[6][2] = {{8, 8}, {16, 8}, {32, 8}, {64, 8}, {64, 16}, {64, 32}};
     int k8 = 0;
     for (auto& c : classes)
         if (c[1] == 8)
-            for (const Plan& p : plans)
-                if (p.triL == c[0] && p.triK == 8) {
+            for (int s = 0; s < n; ++s)
+                if (plans[s].triL == c[0] && plans[s].triK == 8) {
                     ++k8;
                     break;
                 }
     const bool mixed = k8 > 1;
     for (auto& c : classes) {
         const bool inMixed = mixed && c[1] == 8;
-        const int off = (int)hTiles_.size();
-        for (size_t s = 0; s < sims.size(); ++s) {
+        const int off = cnt;
+        for (int s = 0; s < n; ++s) {
             const Plan& p = plans[s];
             if (p.triL != c[0] || p.triK != c[1]) continue;
             const int per = 256 / p.triL;
-            for (int n = 0; n < p.M; n += per)
-                hTiles_.push_back(Tile{(int)s, n, std::min(per, p.M - n), inMixed ? p.triL : 0});
+            for (int m = 0; m < p.M; m += per) put(Tile{s, m, std::min(per, p.M - m), inMixed ? p.triL : 0});
         }
-        const int cnt = (int)hTiles_.size() - off;
-        if (cnt == 0) continue;
-        if (inMixed && !tri_.empty() && tri_.back().L == 0)
-            tri_.back().count += cnt;  // classes are contiguous in the tile list
+        const int num = cnt - off;
+        if (num == 0) continue;
+        if (inMixed && L.ntri > 0 && L.tri[L.ntri - 1].L == 0)
+            L.tri[L.ntri - 1].count += num;  // classes are contiguous in the tile list
         else
-            tri_.push_back(TriClass{inMixed ? 0 : c[0], c[1], off, cnt});
+            L.tri[L.ntri++] = TriClass{inMixed ? 0 : c[0], c[1], off, num};
     }
     // particle tiles: runs of kPartTile particles
     constexpr int kPartTile = 2048;
-    partOff_ = (int)hTiles_.size();
-    for (size_t s = 0; s < sims.size(); ++s)
-        for (int k = 0; k < sims[s].nPart; k += kPartTile)
-            hTiles_.push_back(Tile{(int)s, k, std::min(kPartTile, sims[s].nPart - k), 0});
-    nPart_ = (int)hTiles_.size() - partOff_;
+    L.partOff = cnt;
+    for (int s = 0; s < n; ++s)
+        for (int k = 0; k < sims[s].nPart; k += kPartTile) put(Tile{s, k, std::min(kPartTile, sims[s].nPart - k), 0});
+    L.nPart = cnt - L.partOff;
+    L.total = cnt;
+    return cnt;
+}
+
+static const int kGridCap = 1 << 20;
+
+int launch_tri(const SimDesc* dSims, const Tile* dTiles, const TileLayout& L, int parity,
+               hipStream_t st, bool resid) {
+    // one launch per class (all K = 8 classes of a mixed batch form one, L = 0),
+    // in stream order (side streams would let small classes overlap, but their
+    // fork/join graphs cost far more to capture each time a serving engine's
+    // membership changes)
+    for (int k = 0; k < L.ntri; ++k) {
+        const TriClass& c = L.tri[k];
+        KAM_HIP(launch_tridiag(c.L, c.K, dSims, dTiles + c.off, c.count, parity, kGridCap, st, resid));
+    }
+    return KAM_OK;
+}
+
+int launch_step(const SimDesc* dSims, const Tile* dTiles, const TileLayout& L, int phases, int parity,
+                hipStream_t st, int* newParity, hipEvent_t* ev) {
+    int p = parity;
+    if (L.nsims == 0) {
+        *newParity = p;
+        return KAM_OK;
+    }
+    if (phases & KAM_PHASE_ADVECT) {
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_ADVECT], st));
+        KAM_HIP(launch_advect(dSims, dTiles + L.advOff, L.nAdv, p, L.advStride, L.advLds, kGridCap, st));
+        KAM_HIP(launch_particles(dSims, dTiles + L.partOff, L.nPart, p, kGridCap, st));
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_ADVECT + 1], st));
+        p ^= 1;
+    }
+    if (phases & KAM_PHASE_DIVFFT) {
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT], st));
+        KAM_HIP(launch_div_fft(dSims, dTiles + L.fftOff, L.nFft, p, L.fftLds, kGridCap, st));
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT + 1], st));
+    }
+    if (phases & KAM_PHASE_TRI) {
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_TRI], st));
+        int rc = launch_tri(dSims, dTiles, L, p, st, false);
+        if (rc != KAM_OK) return rc;
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_TRI + 1], st));
+    }
+    if (phases & KAM_PHASE_IFFT) {
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_IFFT], st));
+        KAM_HIP(launch_ifft_grad(dSims, dTiles + L.fftOff, L.nFft32, p, L.fftLds, kGridCap, st, false));
+        KAM_HIP(launch_ifft_grad(dSims, dTiles + L.fftOff + L.nFft32, L.nFft64, p, L.fftLds64, kGridCap, st,
+                                 true));
+        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_IFFT + 1], st));
+    }
+    *newParity = p;
+    return KAM_OK;
+}
+
+int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Plan>& plans,
+                       hipStream_t st) {
+    const int n = (int)plans.size();
+    std::vector<SimDesc> sims(simsIn);
+    std::vector<int> rows(3 * (size_t)n + 1);
+    const int need = plan_tiles(sims.data(), plans.data(), n, rows.data(), nullptr, 0, lay_);
+    hTiles_.resize((size_t)need);
+    plan_tiles(sims.data(), plans.data(), n, rows.data(), hTiles_.data(), need, lay_);
     int rc = reserve(sims.size(), hTiles_.size(), st);
     if (rc != KAM_OK) return rc;
     // the staging buffer is free once the previous upload has landed (long
@@ -477,58 +553,23 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
         KAM_HIP(hipMemcpyAsync(dTiles_, stage + capSims_ * sizeof(SimDesc), tb,
                                hipMemcpyHostToDevice, st));
     KAM_HIP(hipEventRecord(uploaded_, st));
-    nsims_ = (int)sims.size();
     return KAM_OK;
 }
 
-static const int kGridCap = 1 << 20;
+int DeviceBatch::launch_tri_residual(hipStream_t st, int parity) {
+    return launch_tri(dSims_, dTiles_, lay_, parity, st, true);
+}
 
-int DeviceBatch::launch_tri(hipStream_t st, bool resid, int parity) {
-    // one launch per class (all K = 8 classes of a mixed batch form one, L = 0),
-    // in stream order (side streams would let small classes overlap, but their
-    // fork/join graphs cost far more to capture each time a serving engine's
-    // membership changes)
-    for (const TriClass& c : tri_)
-        KAM_HIP(launch_tridiag(c.L, c.K, dSims_, dTiles_ + c.off, c.count, parity, kGridCap, st, resid));
+int DeviceBatch::launch_pressure(hipStream_t st, int parity, float* pout) {
+    const TileLayout& L = lay_;
+    KAM_HIP(launch_ifft_grad(dSims_, dTiles_ + L.fftOff, L.nFft32, parity, L.fftLds, kGridCap, st, false, pout));
+    KAM_HIP(launch_ifft_grad(dSims_, dTiles_ + L.fftOff + L.nFft32, L.nFft64, parity, L.fftLds64, kGridCap, st,
+                             true, pout));
     return KAM_OK;
 }
 
-int DeviceBatch::launch_tri_residual(hipStream_t st, int parity) { return launch_tri(st, true, parity); }
-
-int DeviceBatch::launch(int phases, int parity, hipStream_t st, int* newParity,
-                        hipEvent_t* ev) {
-    if (nsims_ == 0) {
-        *newParity = parity;
-        return KAM_OK;
-    }
-    int p = parity;
-    if (phases & KAM_PHASE_ADVECT) {
-        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_ADVECT], st));
-        KAM_HIP(launch_advect(dSims_, dTiles_ + advOff_, nAdv_, p, advStride_, advLds_, kGridCap, st));
-        KAM_HIP(launch_particles(dSims_, dTiles_ + partOff_, nPart_, p, kGridCap, st));
-        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_ADVECT + 1], st));
-        p ^= 1;
-    }
-    if (phases & KAM_PHASE_DIVFFT) {
-        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT], st));
-        KAM_HIP(launch_div_fft(dSims_, dTiles_ + fftOff_, nFft_, p, fftLds_, kGridCap, st));
-        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT + 1], st));
-    }
-    if (phases & KAM_PHASE_TRI) {
-        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_TRI], st));
-        int rc = launch_tri(st, false, p);
-        if (rc != KAM_OK) return rc;
-        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_TRI + 1], st));
-    }
-    if (phases & KAM_PHASE_IFFT) {
-        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_IFFT], st));
-        KAM_HIP(launch_ifft_grad(dSims_, dTiles_ + fftOff_, nFft32_, p, fftLds_, kGridCap, st, false));
-        KAM_HIP(launch_ifft_grad(dSims_, dTiles_ + fftOff_ + nFft32_, nFft64_, p, fftLds64_, kGridCap, st,
-                                 true));
-        if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_IFFT + 1], st));
-    }
-    *newParity = p;
-    return KAM_OK;
+int DeviceBatch::launch(int phases, int parity, hipStream_t st, int* newParity, hipEvent_t* ev) {
+    return launch_step(dSims_, dTiles_, lay_, phases, parity, st, newParity, ev);
 }
 
 // ---------------------------------------------------------------------------
@@ -581,6 +622,7 @@ struct kam_sim {
     int parity = 0;
     float* pmem = nullptr;  // particles: 2 copies of nPart float2
     double* resid = nullptr;  // per-mode residual report, on first use
+    float* pbuf = nullptr;    // pressure readback, on first use
 };
 
 extern "C" {
@@ -610,9 +652,9 @@ static int check_desc(const kam_sim_desc* d, Plan& plan) {
     if (!d) return set_err(KAM_EINVAL, "null descriptor");
     if (!(d->dt > 0.0) || !std::isfinite(d->dt)) return set_err(KAM_EINVAL, "dt must be > 0");
     if (!(d->radius > 0.0) || !std::isfinite(d->radius)) return set_err(KAM_EINVAL, "radius must be > 0");
-    if (d->precision != KAM_PREC_FP32 && d->precision != KAM_PREC_FP64_PRESSURE)
-        return set_err(KAM_EINVAL, "unknown precision %d", d->precision);
-    return make_plan(d->n_theta, d->n_phi, plan);
+    int rc = make_plan(d->n_theta, d->n_phi, plan);
+    if (rc != KAM_OK) return rc;
+    return check_precision(d->precision, plan);
 }
 
 int kam_sim_create(const kam_sim_desc* d, kam_sim** out) {
@@ -664,6 +706,7 @@ int kam_sim_destroy(kam_sim* s) {
     if (s->mem) (void)hipFree(s->mem);
     if (s->pmem) (void)hipFree(s->pmem);
     if (s->resid) (void)hipFree(s->resid);
+    if (s->pbuf) (void)hipFree(s->pbuf);
     if (s->tables.rowf) release_tables(s->sd.device, s->sd.n_theta, s->sd.dt, s->sd.radius);
     delete s;
     return KAM_OK;
@@ -773,6 +816,22 @@ int kam_sim_poisson_residual(kam_sim* s, double* per_mode, double* max_rel, doub
     if (per_mode) std::memcpy(per_mode, h.data(), (size_t)M * sizeof(double));
     if (max_rel) *max_rel = mx;
     if (rms_rel) *rms_rel = std::sqrt(sq / M);
+    return KAM_OK;
+}
+
+int kam_sim_get_pressure(kam_sim* s, float* p) {
+    if (!s || !p) return set_err(KAM_EINVAL, "null argument");
+    KAM_HIP(hipSetDevice(s->sd.device));
+    const size_t N = (size_t)s->plan.nT * s->plan.nP;
+    if (!s->pbuf) {
+        KAM_HIP(hipStreamSynchronize(s->stream));
+        if (hipMalloc(&s->pbuf, N * sizeof(float)) != hipSuccess)
+            return set_err(KAM_ENOMEM, "pressure buffer allocation failed");
+    }
+    int rc = s->batch.launch_pressure(s->stream, s->parity, s->pbuf);
+    if (rc != KAM_OK) return rc;
+    KAM_HIP(hipMemcpyAsync(p, s->pbuf, N * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    KAM_HIP(hipStreamSynchronize(s->stream));
     return KAM_OK;
 }
 

@@ -31,7 +31,7 @@ def test_library_exports_every_declared_symbol():
 def test_bad_shapes_are_rejected_without_a_gpu():
     L = K.lib()
     h = C.c_void_p()
-    for nt, nph, dt in [(3, 6, 0.1), (64, 100, 0.1), (64, 128, 0.0), (7 * 11, 154, 0.1), (2048, 4096, 0.1)]:
+    for nt, nph, dt in [(3, 6, 0.1), (64, 100, 0.1), (64, 128, 0.0), (7 * 11, 154, 0.1), (4096, 8192, 0.1)]:
         d = K.SimDesc(nt, nph, 1.0, dt, 1, 0, 0)
         rc = L.kam_sim_create(C.byref(d), C.byref(h))
         assert rc != K.KAM_OK
@@ -73,3 +73,4 @@ def test_tuning_table_round_trip_without_a_gpu():
     assert K.tune_get(192) == default
     with pytest.raises(K.KaminoError):
         K.tune_set(2, 1, 1, 1)
+
