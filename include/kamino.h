@@ -208,6 +208,27 @@ size_t kam_sim_bytes(int n_theta, int n_phi);
 int kam_init_fields(int n_theta, int n_phi, double radius, int L, uint64_t seed,
                     int density_kind, float *u_phi, float *u_theta, float *rho);
 
+/* ==== Stepping without allocation ========================================
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  Runs nsteps full steps (as kam_sim_step, the same
+ * kernels) of one simulation whose fields are DEVICE buffers owned by the
+ * caller, dense theta-major as above (u_theta has nTheta - 1 rows), updated
+ * in place; rho may be NULL (zero density, not written back).  Work is
+ * queued on `stream` (a hipStream_t; NULL = the null stream).  Intermediate
+ * state, the spectrum, tables and tile lists live in one fixed scratch array
+ * per device of KAM_NOALLOC_SCRATCH_BYTES, which each device that runs the
+ * library holds until the process ends; shapes that do not fit (n_theta
+ * above about 512) return KAM_EINVAL.  The call allocates nothing: no
+ * hipMalloc / hipFree and no host heap.  The first call for a shape on a
+ * device uploads its tables and waits for that upload on `stream`; later
+ * calls with the same shape queue only device work.  Every call on a device
+ * shares its scratch array: calls that overlap on one device (other threads,
+ * or other streams with work still queued) race. */
+#define KAM_NOALLOC_SCRATCH_BYTES (16u << 20)
+int kam_sim_step_noalloc(const kam_sim_desc *desc, float *u_phi, float *u_theta, float *rho,
+                         int nsteps, void *stream);
+
 /* ==== Batching and serving engine ========================================
  * No counterpart in the reference, which runs one simulation per process
  * (KaminoSolver.cu:20 hard-codes device 0 and the dtor resets the device).

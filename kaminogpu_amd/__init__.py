@@ -37,8 +37,9 @@ ABI_SYMBOLS = [
     "kam_engine_create", "kam_engine_destroy", "kam_engine_admit", "kam_engine_step",
     "kam_engine_poll", "kam_engine_fetch", "kam_engine_cancel", "kam_engine_sync",
     "kam_engine_stats", "kam_engine_reset_stats", "kam_engine_profile", "kam_engine_now",
-    "kam_sim_get_pressure",
+    "kam_sim_step_noalloc", "kam_sim_get_pressure",
 ]
+NOALLOC_SCRATCH_BYTES = 16 << 20  # KAM_NOALLOC_SCRATCH_BYTES
 
 
 class KaminoError(RuntimeError):
@@ -98,6 +99,8 @@ def lib():
     L.kam_sim_poisson_residual.argtypes = [C.c_void_p, _dpp, _dpp, _dpp]
     L.kam_image_density.argtypes = [C.c_char_p, C.c_int, C.c_int, _fp]
     L.kam_image_colors.argtypes = [C.c_char_p, C.c_int, C.c_int, _fp, C.c_int64, _fp]
+    L.kam_sim_step_noalloc.argtypes = [C.POINTER(SimDesc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                       C.c_void_p]
     _bind_engine(L)
     _lib = L
     return L

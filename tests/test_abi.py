@@ -74,3 +74,13 @@ def test_tuning_table_round_trip_without_a_gpu():
     with pytest.raises(K.KaminoError):
         K.tune_set(2, 1, 1, 1)
 
+
+def test_noalloc_step_rejects_bad_arguments_without_a_gpu():
+    L = K.lib()
+    d = K.SimDesc(64, 128, 1.0, 0.01, 1, 0, 0)
+    assert L.kam_sim_step_noalloc(None, 1, 1, None, 1, None) == -1  # KAM_EINVAL
+    assert L.kam_sim_step_noalloc(C.byref(d), None, 1, None, 1, None) == -1
+    assert L.kam_sim_step_noalloc(C.byref(d), 1, 1, None, -1, None) == -1
+    bad = K.SimDesc(64, 100, 1.0, 0.01, 1, 0, 0)
+    assert L.kam_sim_step_noalloc(C.byref(bad), 1, 1, None, 1, None) == -1
+    assert L.kam_last_error().decode()
