@@ -130,7 +130,10 @@ def tune(K, nt, batch, cache):
     if nt not in TUNED:
         ms = K.autotune(nt, batch, steps=10, cache_path=cache)
         TUNED.add(nt)
-    return dict(zip(["adv_rows", "adv_cols", "fft_rows"], K.tune_get(nt))), ms
+    tiles = dict(zip(["adv_rows", "adv_cols", "fft_rows"], K.tune_get(nt)))
+    # the phi transform the autotuner picked for this size: the row FFT or the MFMA DFT
+    tiles["transform"] = "dft_mfma" if K.tune_get_transform(nt) == K.TRANSFORM_DFT else "fft"
+    return tiles, ms
 
 
 def run_uneven(K, steps, cache, sims=64, seed=3):

@@ -42,6 +42,13 @@ enum {
     KAM_PHASE_ALL = 1 | 8 | 16 | 32
 };
 
+/* ---- phi transforms (kam_tune_set_transform) ----------------------------- */
+enum {
+    KAM_TRANSFORM_FFT = 0,  /* the row FFT in LDS (default) */
+    KAM_TRANSFORM_DFT = 1   /* dense DFT products on MFMA: FP16 hi/lo operands, FP32
+                               accumulate (FP32-pressure simulations, n_theta <= 1024) */
+};
+
 /* ---- precision modes ----------------------------------------------------- */
 enum {
     KAM_PREC_FP32 = 0,          /* FP32 fields, advection and FFTs; FP64 tridiagonal solves */
@@ -329,11 +336,22 @@ int kam_tune_set(int n_theta, int adv_rows, int adv_cols, int fft_rows);
 /* The shapes a simulation of n_theta would get now. */
 int kam_tune_get(int n_theta, int *adv_rows, int *adv_cols, int *fft_rows);
 
+/* Records the phi transform (KAM_TRANSFORM_*) for simulations of n_theta
+ * created or admitted from now on; kam_autotune picks it by timing both.
+ * The reference's transform is cuFFT (KaminoCore.cu:763-764, 812-813;
+ * plan KaminoSolver.cu:61-66).  The transform never changes results beyond
+ * rounding (both are FP32-accurate). */
+int kam_tune_set_transform(int n_theta, int transform);
+int kam_tune_get_transform(int n_theta, int *transform);
+
 /* Times candidate shapes on `batch` synthetic simulations of n_theta (steps
- * timed steps each), records the fastest with kam_tune_set, appends a line
- * "gfx950 n_theta batch adv_rows adv_cols fft_rows adv_ms fft_ms" to
- * cache_path (may be NULL), and returns the best per-kernel times in
- * kernel_ms[4] (advection, div+FFT, tridiagonal, inverse FFT+gradient). */
+ * timed steps each): advection tiles, FFT tile heights, then the row FFT
+ * against the MFMA DFT (n_theta <= 1024).  Records the fastest with
+ * kam_tune_set and kam_tune_set_transform, appends a line
+ * "gfx950 n_theta batch adv_rows adv_cols fft_rows adv_ms fft_ms transform"
+ * to cache_path (may be NULL), and returns the best per-kernel times in
+ * kernel_ms[4] (advection, div+transform, tridiagonal, inverse
+ * transform+gradient). */
 int kam_autotune(int device, int n_theta, int batch, int steps, const char *cache_path,
                  double *kernel_ms);
 

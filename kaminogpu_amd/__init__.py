@@ -37,8 +37,9 @@ ABI_SYMBOLS = [
     "kam_engine_create", "kam_engine_destroy", "kam_engine_admit", "kam_engine_step",
     "kam_engine_poll", "kam_engine_fetch", "kam_engine_cancel", "kam_engine_sync",
     "kam_engine_stats", "kam_engine_reset_stats", "kam_engine_profile", "kam_engine_now",
-    "kam_sim_step_noalloc", "kam_sim_get_pressure",
+    "kam_sim_step_noalloc", "kam_sim_get_pressure", "kam_tune_set_transform", "kam_tune_get_transform",
 ]
+TRANSFORM_FFT, TRANSFORM_DFT = 0, 1
 NOALLOC_SCRATCH_BYTES = 16 << 20  # KAM_NOALLOC_SCRATCH_BYTES
 
 
@@ -89,6 +90,8 @@ def lib():
     L.kam_tune_get.argtypes = [C.c_int, _ip, _ip, _ip]
     L.kam_autotune.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_char_p, _dpp]
     L.kam_tune_load.argtypes = [C.c_char_p]
+    L.kam_tune_set_transform.argtypes = [C.c_int, C.c_int]
+    L.kam_tune_get_transform.argtypes = [C.c_int, C.POINTER(C.c_int)]
     L.kam_init_fields.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, C.c_int, _fp, _fp, _fp]
     L.kam_sim_set_particles.argtypes = [C.c_void_p, _fp, C.c_int64]
     L.kam_sim_get_particles.argtypes = [C.c_void_p, _fp, C.c_int64]
@@ -143,6 +146,17 @@ def tune_set(n_theta, adv_rows=0, adv_cols=0, fft_rows=0):
     check(lib().kam_tune_set(int(n_theta), int(adv_rows), int(adv_cols), int(fft_rows)))
 
 
+def tune_set_transform(n_theta, transform):
+    """Phi transform for simulations of n_theta created from now on (TRANSFORM_FFT / TRANSFORM_DFT)."""
+    check(lib().kam_tune_set_transform(int(n_theta), int(transform)))
+
+
+def tune_get_transform(n_theta):
+    t = C.c_int()
+    check(lib().kam_tune_get_transform(int(n_theta), C.byref(t)))
+    return t.value
+
+
 def autotune(n_theta, batch, steps=10, cache_path=None, device=0):
     """Times candidate tile shapes on the device and keeps the fastest;
     returns the best per-kernel times (ms)."""
@@ -159,7 +173,7 @@ def tune_load(cache_path):
     with open(cache_path) as f:
         for line in f:
             parts = line.split()
-            if len(parts) == 8 and parts[0] == "gfx950":
+            if len(parts) in (8, 9) and parts[0] == "gfx950":
                 sizes.add(int(parts[1]))
     return sizes
 

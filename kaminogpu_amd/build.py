@@ -14,8 +14,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libkamino.so")
 DRIVER = os.path.join(PKG, "bin", "kamino")
 
-HIP_SOURCES = ["kamino_kernels.hip", "kamino_runtime.hip", "kamino_engine.hip", "kamino_noalloc.hip",
-               "kamino_init.cpp"]
+HIP_SOURCES = ["kamino_kernels.hip", "kamino_dft.hip", "kamino_runtime.hip", "kamino_engine.hip",
+               "kamino_noalloc.hip", "kamino_init.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # gfx950 only; packed FP32 instructions are disabled for every HIP source.
 HIP_FLAGS = [

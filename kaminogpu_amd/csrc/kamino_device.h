@@ -46,7 +46,8 @@ enum RowTabD {
 
 enum SimFlags {
     SIM_GEOMETRIC = 1,
-    SIM_P64 = 2  // FP64 pressure: p-hat stored, transformed and differentiated in FP64
+    SIM_P64 = 2,  // FP64 pressure: p-hat stored, transformed and differentiated in FP64
+    SIM_DFT = 4   // phi transforms as dense DFTs on MFMA (FP16 hi/lo matrices) instead of FFTs
 };
 
 struct SimDesc {
@@ -79,6 +80,12 @@ struct SimDesc {
     int nPart;             // particle count (0: none)
     float radius, dt;      // FP32, as the reference's constants (KaminoCore.cu:868-872)
     double* resid;         // per-mode Poisson residual report (diagnostic launches only)
+    // SIM_DFT: FP16 bit patterns of the forward and inverse DFT matrices, hi and
+    // lo planes, each [N][dftK] (N = nPhi rounded up to 128 output columns,
+    // dftK = nPhi rounded up to 32), scaled by 2^13
+    const uint16_t* dftFwd[2];
+    const uint16_t* dftInv[2];
+    int dftK;
 };
 
 // One unit of work: rows [row0, row0 + rows) of simulation `sim` (or, for the

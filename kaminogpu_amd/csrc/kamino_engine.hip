@@ -213,6 +213,8 @@ int rebuild(kam_engine* e) {
             r.desc.advRows = fresh.advRows;
             r.desc.fftRows = fresh.fftRows;
             set_tile_cols(r.desc, fresh.advCols);
+            int rc = attach_transform(r.desc, fresh, r.sd.precision, e->ed.device);
+            if (rc != KAM_OK) return rc;
         }
         sims.push_back(r.desc);
         plans.push_back(r.plan);
@@ -494,6 +496,13 @@ int kam_engine_admit(kam_engine* e, const kam_sim_desc* d, const float* u_phi,
     e->freeSlots.pop_back();
     float* mem = reinterpret_cast<float*>(e->arena.base() + r.off);
     fill_desc(r.desc, r.plan, r.tables, *d, mem);
+    rc = attach_transform(r.desc, r.plan, d->precision, e->ed.device);
+    if (rc != KAM_OK) {
+        e->freeSlots.push_back(r.slot);
+        e->arena.release(r.off, r.bytes);
+        release_tables(e->ed.device, d->n_theta, d->dt, d->radius);
+        return rc;
+    }
     r.desc.status = e->dStatus + r.slot;
     KAM_HIP(hipMemsetAsync(r.desc.status, 0, sizeof(int), e->stream));
     const size_t N = (size_t)d->n_theta * d->n_phi;
@@ -690,6 +699,21 @@ int kam_tune_set(int n_theta, int adv_rows, int adv_cols, int fft_rows) {
     return tune_set(n_theta, adv_rows, adv_cols, fft_rows);
 }
 
+int kam_tune_set_transform(int n_theta, int transform) {
+    if (n_theta < 4 || (transform != KAM_TRANSFORM_FFT && transform != KAM_TRANSFORM_DFT))
+        return set_err(KAM_EINVAL, "bad transform entry");
+    return tune_set_transform(n_theta, transform);
+}
+
+int kam_tune_get_transform(int n_theta, int* transform) {
+    if (!transform) return set_err(KAM_EINVAL, "null argument");
+    Plan p;
+    int rc = make_plan(n_theta, 2 * n_theta, p);
+    if (rc != KAM_OK) return rc;
+    *transform = p.transform;
+    return KAM_OK;
+}
+
 int kam_tune_get(int n_theta, int* adv_rows, int* adv_cols, int* fft_rows) {
     Plan p;
     int rc = make_plan(n_theta, 2 * n_theta, p);
@@ -760,7 +784,22 @@ int kam_autotune(int device, int n_theta, int batch, int steps, const char* cach
             bestF = fftC[fi];
         }
     }
+    // the phi transform: the row FFT (just timed) against the MFMA DFT
+    int bestT = KAM_TRANSFORM_FFT;
+    if (rc == KAM_OK && n_theta <= 1024) {
+        tune_set(n_theta, bestR, bestW, bestF);
+        tune_set_transform(n_theta, KAM_TRANSFORM_DFT);
+        e->dirty = true;
+        if ((rc = kam_engine_profile(e, 2, k)) == KAM_OK && (rc = kam_engine_profile(e, steps, k)) == KAM_OK &&
+            k[KC_DIVFFT] + k[KC_IFFT] < best[KC_DIVFFT] + best[KC_IFFT]) {
+            best[KC_DIVFFT] = k[KC_DIVFFT];
+            best[KC_IFFT] = k[KC_IFFT];
+            best[KC_TRI] = k[KC_TRI];
+            bestT = KAM_TRANSFORM_DFT;
+        }
+    }
     kam_engine_destroy(e);
+    tune_set_transform(n_theta, bestT);
     if (rc != KAM_OK) return rc;
     tune_set(n_theta, bestR, bestW, bestF);
     if (kernel_ms)
@@ -768,8 +807,8 @@ int kam_autotune(int device, int n_theta, int batch, int steps, const char* cach
     if (cache_path && *cache_path) {
         FILE* f = std::fopen(cache_path, "a");
         if (!f) return set_err(KAM_EINVAL, "cannot append to %s", cache_path);
-        std::fprintf(f, "gfx950 %d %d %d %d %d %.5f %.5f\n", n_theta, batch, bestR, bestW, bestF,
-                     best[KC_ADVECT], best[KC_DIVFFT] + best[KC_IFFT]);
+        std::fprintf(f, "gfx950 %d %d %d %d %d %.5f %.5f %d\n", n_theta, batch, bestR, bestW, bestF,
+                     best[KC_ADVECT], best[KC_DIVFFT] + best[KC_IFFT], bestT);
         std::fclose(f);
     }
     return KAM_OK;
@@ -779,13 +818,16 @@ int kam_tune_load(const char* cache_path) {
     if (!cache_path) return set_err(KAM_EINVAL, "null path");
     FILE* f = std::fopen(cache_path, "r");
     if (!f) return set_err(KAM_ENOTFOUND, "no tuning cache at %s", cache_path);
-    char arch[32];
-    int nT, b, R, W, F;
+    char line[256], arch[32];
+    int nT, b, R, W, F, T;
     double t1, t2;
     int n = 0;
-    while (std::fscanf(f, "%31s %d %d %d %d %d %lf %lf", arch, &nT, &b, &R, &W, &F, &t1, &t2) == 8) {
-        if (std::strcmp(arch, "gfx950") == 0 && nT >= 4) {
+    // lines of 8 fields (written before the transform choice) or 9
+    while (std::fgets(line, sizeof line, f)) {
+        const int got = std::sscanf(line, "%31s %d %d %d %d %d %lf %lf %d", arch, &nT, &b, &R, &W, &F, &t1, &t2, &T);
+        if (got >= 8 && std::strcmp(arch, "gfx950") == 0 && nT >= 4) {
             tune_set(nT, R, W, F);  // later lines win
+            tune_set_transform(nT, got == 9 ? T : KAM_TRANSFORM_FFT);
             ++n;
         }
     }

@@ -15,6 +15,7 @@
 #include <type_traits>
 
 #include "kamino_device.h"
+#include "kamino_devutil.h"
 #include "kamino_launch.h"
 
 namespace kam {
@@ -29,35 +30,6 @@ enum Kind { K_VPHI = 0, K_VTHETA = 1, K_CENTERED = 2 };
 // Fields are read through buffer resources (32-bit offsets, hardware range
 // check), and the branch choice is made with selects, not jumps.
 // ---------------------------------------------------------------------------
-typedef __amdgpu_buffer_rsrc_t Rsrc;
-
-__device__ __forceinline__ Rsrc make_rsrc(const float* p, int nfloats) {
-    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, nfloats * 4, 0x00020000);
-}
-__device__ __forceinline__ float ldf(Rsrc r, int elem) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, elem * 4, 0, 0));
-}
-__device__ __forceinline__ float2 ldf2(Rsrc r, int elem) {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 4, 0, 0));
-}
-__device__ __forceinline__ double ldd(Rsrc r, int elem) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, elem * 8, 0, 0));
-}
-__device__ __forceinline__ float4 ldf4(Rsrc r, int elem) {
-    return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 4, 0, 0));
-}
-__device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
-
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Block b takes tile (b % 8) * per + b / 8, so every XCD walks a contiguous
-// run of the tile list and neighbouring tiles, which share halo rows and
-// spectrum lines, meet in the same L2.  Launches use 8 * per blocks.
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_tile(int ntiles) {
-    const int per = (ntiles + kXcds - 1) / kXcds;
-    return (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
-}
-
 // Per-shape constants a sampler needs.
 struct Grid {
     int nT, nP, half;
@@ -126,12 +98,6 @@ __device__ __forceinline__ float sample(Rsrc f, const Grid& g, int bi, float fph
         b1 = ldf(f, oB + cB1);
     }
     return lerpf(lerpf(a0, a1, ap), lerpf(b0, b1, ap), w);
-}
-
-// Watchdog: any non-finite result of a step marks the simulation (a vector
-// atomic; taken only on the rare failing path).
-__device__ __forceinline__ void flag_bad(int* status, bool bad) {
-    if (bad && status) atomicOr(status, 1);
 }
 
 // Node offsets of each grid in cells (KaminoHeader.cuh:30-41) and the shift
@@ -1078,13 +1044,24 @@ __device__ __forceinline__ double rcp64(double x) {
     return fma(r, fma(-x, r, 1.0), r);  // one Newton step doubles the hardware estimate's bits
 }
 
-template <int L>
-__device__ __forceinline__ double shfl_grp(double v, int srcLaneInGroup, int laneBase) {
+// 1/x to FP32 precision (the hardware estimate is within 1 ulp).
+__device__ __forceinline__ float rcpT(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcpT(double x) { return rcp64(x); }
+
+template <int L, typename T>
+__device__ __forceinline__ T shfl_grp(T v, int srcLaneInGroup, int laneBase) {
     return __shfl(v, laneBase + srcLaneInGroup, 64);
 }
 
+// With the FP32 elimination, the tiles holding the low modes (below
+// kTriFp64Modes) add one pass of iterative refinement with an FP64 residual:
+// their systems are the least diagonally dominant (FP32 Thomas loses up to
+// 1e-4 relative at mode 1 for n_theta = 1024, ~6e-8, FP32 rounding, from
+// mode 16 on; one refinement pass brings mode 1 to ~2e-8).
+
 // One tile of the tridiagonal solve: G = 256 / L modes of one simulation,
-// L lanes per mode.  The block's G mode columns and the FP64 row terms pass
+// L lanes per mode; T is the arithmetic of the elimination (the coefficients
+// are formed in FP64 and rounded to T).  The block's G mode columns and the FP64 row terms pass
 // through LDS (zs: G x KMAX x (L+1) float2; cs, ss: KMAX x (L+1) doubles):
 // global reads and writes stay contiguous, and lane l finds its row l K + k
 // at [k][l] (a padded row per k keeps the staging writes spread over the
@@ -1096,7 +1073,7 @@ __device__ __forceinline__ double shfl_grp(double v, int srcLaneInGroup, int lan
 // write-back.
 constexpr int kTriBatch = 4;  // float4 spectrum loads per thread in flight while staging
 
-template <int L, int KMAX, bool RESID>
+template <int L, int KMAX, bool RESID, typename T>
 __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const Tile& tile,
                                          int parity, float2* zsp, double* csp, double* ssp) {
     constexpr int G = 256 / L;  // modes per block
@@ -1174,191 +1151,244 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
         __syncthreads();
         const double h2 = s.invH2;
         const double nn = (n == 0) ? (double)s.M * (double)s.M : (double)n * (double)n;
-
-        double al[KMAX], ga[KMAX], dr[KMAX], di[KMAX];
+        // row i = r0 + k of this lane, a x_{i-1} + b x_i + c x_{i+1} = f, in FP64
+        // (the pole rows folded, KaminoSolver.cu:117-163)
+        auto coef = [&](int k, double& a, double& b, double& c) {
+            const int i = r0 + k;
+            const double ct = cs(k, l);
+            a = h2 - ct;
+            c = h2 + ct;
+            b = -2.0 * h2 - nn * ss(k, l);
+            if (i == 0) {
+                b += a;
+                a = 0.0;
+            }
+            if (i == nT - 1) {
+                b += c;
+                c = 0.0;
+            }
+        };
+        // Solves the group's system for the right-hand side rhs(k) (a float2 per
+        // row of this lane) into (xr, xi), in T: Thomas within the lane, then
+        // parallel cyclic reduction of the lanes' last unknowns.  ACC adds the
+        // solution to (xr, xi) instead (a refinement's correction).
+        auto solve = [&](auto accTag, auto rhs, T* xr, T* xi) {
+            constexpr bool ACC = decltype(accTag)::value;
+            T al[KMAX], ga[KMAX], dr[KMAX], di[KMAX];
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            al[k] = 0.0;
-            ga[k] = 0.0;
-            dr[k] = 0.0;
-            di[k] = 0.0;
-        }
-        // forward elimination within the lane's rows
+            for (int k = 0; k < KMAX; ++k) {
+                al[k] = T(0);
+                ga[k] = T(0);
+                dr[k] = T(0);
+                di[k] = T(0);
+            }
+            // forward elimination within the lane's rows
 #pragma unroll
-        for (int k = 0; k < KMAX; ++k) {
-            if (k < cnt) {
-                const int i = r0 + k;
-                const float2 d = zs(g, k, l);
-                const double dre = (n == 0) ? 0.0 : (double)d.x;
-                const double dim = (double)d.y;
-                const double ct = cs(k, l);
-                double a = h2 - ct, c = h2 + ct;
-                double b = -2.0 * h2 - nn * ss(k, l);
-                if (i == 0) {
-                    b += a;
-                    a = 0.0;
-                }
-                if (i == nT - 1) {
-                    b += c;
-                    c = 0.0;
-                }
-                if (k == 0) {
-                    const double inv = rcp64(b);
-                    al[k] = a * inv;
-                    ga[k] = c * inv;
-                    dr[k] = dre * inv;
-                    di[k] = dim * inv;
-                } else {
-                    const double inv = rcp64(b - a * ga[k - 1]);
-                    al[k] = -a * al[k - 1] * inv;
-                    ga[k] = c * inv;
-                    dr[k] = (dre - a * dr[k - 1]) * inv;
-                    di[k] = (dim - a * di[k - 1]) * inv;
+            for (int k = 0; k < KMAX; ++k) {
+                if (k < cnt) {
+                    double a64, b64, c64;
+                    coef(k, a64, b64, c64);
+                    const T a = (T)a64, b = (T)b64, c = (T)c64;
+                    const float2 f = rhs(k);
+                    const T fr = (T)f.x, fi = (T)f.y;
+                    if (k == 0) {
+                        const T inv = rcpT(b);
+                        al[k] = a * inv;
+                        ga[k] = c * inv;
+                        dr[k] = fr * inv;
+                        di[k] = fi * inv;
+                    } else {
+                        const T inv = rcpT(b - a * ga[k - 1]);
+                        al[k] = -a * al[k - 1] * inv;
+                        ga[k] = c * inv;
+                        dr[k] = (fr - a * dr[k - 1]) * inv;
+                        di[k] = (fi - a * di[k - 1]) * inv;
+                    }
                 }
             }
-        }
-        // x_s = P + Q x_{s-1} + S x_e, by backward recurrence from the last row
-        double Pr = 0.0, Pi = 0.0, Q = 0.0, S = 1.0;
+            // x_s = P + Q x_{s-1} + S x_e, by backward recurrence from the last row
+            T Pr = T(0), Pi = T(0), Q = T(0), S = T(1);
 #pragma unroll
-        for (int k = KMAX - 2; k >= 0; --k) {
-            if (k < cnt - 1) {
-                Pr = dr[k] - ga[k] * Pr;
-                Pi = di[k] - ga[k] * Pi;
-                Q = -al[k] - ga[k] * Q;
-                S = -ga[k] * S;
+            for (int k = KMAX - 2; k >= 0; --k) {
+                if (k < cnt - 1) {
+                    Pr = dr[k] - ga[k] * Pr;
+                    Pi = di[k] - ga[k] * Pi;
+                    Q = -al[k] - ga[k] * Q;
+                    S = -ga[k] * S;
+                }
             }
-        }
-        // reduced equation of this lane: A X_{l-1} + B X_l + C X_{l+1} = D
-        const int src = (l + 1 < L) ? l + 1 : l;
-        const double nPr = shfl_grp<L>(Pr, src, base), nPi = shfl_grp<L>(Pi, src, base);
-        const double nQ = shfl_grp<L>(Q, src, base), nS = shfl_grp<L>(S, src, base);
-        double A = 0.0, B = 1.0, C = 0.0, Dr = 0.0, Di = 0.0;
-        if (cnt > 0) {
-            double gE = 0.0, aE = 0.0, drE = 0.0, diE = 0.0;
+            // reduced equation of this lane: A X_{l-1} + B X_l + C X_{l+1} = D
+            const int src = (l + 1 < L) ? l + 1 : l;
+            const T nPr = shfl_grp<L>(Pr, src, base), nPi = shfl_grp<L>(Pi, src, base);
+            const T nQ = shfl_grp<L>(Q, src, base), nS = shfl_grp<L>(S, src, base);
+            T A = T(0), B = T(1), C = T(0), Dr = T(0), Di = T(0);
+            if (cnt > 0) {
+                T gE = T(0), aE = T(0), drE = T(0), diE = T(0);
+#pragma unroll
+                for (int k = 0; k < KMAX; ++k)
+                    if (k == cnt - 1) {
+                        gE = ga[k];
+                        aE = al[k];
+                        drE = dr[k];
+                        diE = di[k];
+                    }
+                const bool hasNext = (l + 1 < L);
+                A = aE;
+                B = T(1) + (hasNext ? gE * nQ : T(0));
+                C = hasNext ? gE * nS : T(0);
+                Dr = drE - (hasNext ? gE * nPr : T(0));
+                Di = diE - (hasNext ? gE * nPi : T(0));
+            }
+            // parallel cyclic reduction across the L lanes of the group
+#pragma unroll
+            for (int st = 1; st < L; st <<= 1) {
+                const int lm = l - st, lp = l + st;
+                const int sm = lm >= 0 ? lm : l, sp = lp < L ? lp : l;
+                T Am = shfl_grp<L>(A, sm, base), Bm = shfl_grp<L>(B, sm, base);
+                T Cm = shfl_grp<L>(C, sm, base), Dmr = shfl_grp<L>(Dr, sm, base);
+                T Dmi = shfl_grp<L>(Di, sm, base);
+                T Ap = shfl_grp<L>(A, sp, base), Bp = shfl_grp<L>(B, sp, base);
+                T Cp = shfl_grp<L>(C, sp, base), Dpr = shfl_grp<L>(Dr, sp, base);
+                T Dpi = shfl_grp<L>(Di, sp, base);
+                if (lm < 0) {
+                    Am = T(0);
+                    Bm = T(1);
+                    Cm = T(0);
+                    Dmr = T(0);
+                    Dmi = T(0);
+                }
+                if (lp >= L) {
+                    Ap = T(0);
+                    Bp = T(1);
+                    Cp = T(0);
+                    Dpr = T(0);
+                    Dpi = T(0);
+                }
+                const T k1 = A * rcpT(Bm), k2 = C * rcpT(Bp);
+                const T nA = -Am * k1, nC = -Cp * k2;
+                const T nB = B - Cm * k1 - Ap * k2;
+                const T nDr = Dr - Dmr * k1 - Dpr * k2;
+                const T nDi = Di - Dmi * k1 - Dpi * k2;
+                A = nA;
+                B = nB;
+                C = nC;
+                Dr = nDr;
+                Di = nDi;
+            }
+            const T iB = rcpT(B);
+            const T Xr = Dr * iB, Xi = Di * iB;
+            const int sprev = (l > 0) ? l - 1 : l;
+            T xmr = shfl_grp<L>(Xr, sprev, base), xmi = shfl_grp<L>(Xi, sprev, base);
+            if (l == 0) {
+                xmr = T(0);
+                xmi = T(0);
+            }
+            // back substitution: x_i = delta_i - alpha_i x_{s-1} - gamma_i x_{i+1}
+            T vr = Xr, vi = Xi;
+#pragma unroll
+            for (int k = KMAX - 1; k >= 0; --k) {
+                if (!ACC) {
+                    xr[k] = T(0);
+                    xi[k] = T(0);
+                }
+                if (k < cnt) {
+                    if (k < cnt - 1) {
+                        vr = dr[k] - al[k] * xmr - ga[k] * vr;
+                        vi = di[k] - al[k] * xmi - ga[k] * vi;
+                    }
+                    xr[k] = ACC ? xr[k] + vr : vr;
+                    xi[k] = ACC ? xi[k] + vi : vi;
+                }
+            }
+        };
+        // r = f - T x in FP64, f the staged right-hand side (FP32 values, exact),
+        // one row at a time through `use(k, er, ei, fre, fim)`; the
+        // neighbouring unknowns across lane boundaries come by shuffles
+        auto residual = [&](const T* xr, const T* xi, auto use) {
+            const int sprev = (l > 0) ? l - 1 : l, snext = (l + 1 < L) ? l + 1 : l;
+            T lastR = T(0), lastI = T(0);
 #pragma unroll
             for (int k = 0; k < KMAX; ++k)
                 if (k == cnt - 1) {
-                    gE = ga[k];
-                    aE = al[k];
-                    drE = dr[k];
-                    diE = di[k];
+                    lastR = xr[k];
+                    lastI = xi[k];
                 }
-            const bool hasNext = (l + 1 < L);
-            A = aE;
-            B = 1.0 + (hasNext ? gE * nQ : 0.0);
-            C = hasNext ? gE * nS : 0.0;
-            Dr = drE - (hasNext ? gE * nPr : 0.0);
-            Di = diE - (hasNext ? gE * nPi : 0.0);
-        }
-        // parallel cyclic reduction across the L lanes of the group
-#pragma unroll
-        for (int st = 1; st < L; st <<= 1) {
-            const int lm = l - st, lp = l + st;
-            const int sm = lm >= 0 ? lm : l, sp = lp < L ? lp : l;
-            double Am = shfl_grp<L>(A, sm, base), Bm = shfl_grp<L>(B, sm, base);
-            double Cm = shfl_grp<L>(C, sm, base), Dmr = shfl_grp<L>(Dr, sm, base);
-            double Dmi = shfl_grp<L>(Di, sm, base);
-            double Ap = shfl_grp<L>(A, sp, base), Bp = shfl_grp<L>(B, sp, base);
-            double Cp = shfl_grp<L>(C, sp, base), Dpr = shfl_grp<L>(Dr, sp, base);
-            double Dpi = shfl_grp<L>(Di, sp, base);
-            if (lm < 0) {
-                Am = 0.0;
-                Bm = 1.0;
-                Cm = 0.0;
-                Dmr = 0.0;
-                Dmi = 0.0;
+            double pmr = (double)shfl_grp<L>(lastR, sprev, base), pmi = (double)shfl_grp<L>(lastI, sprev, base);
+            const double x0r = (double)shfl_grp<L>(xr[0], snext, base);
+            const double x0i = (double)shfl_grp<L>(xi[0], snext, base);
+            if (l == 0) {
+                pmr = 0.0;
+                pmi = 0.0;
             }
-            if (lp >= L) {
-                Ap = 0.0;
-                Bp = 1.0;
-                Cp = 0.0;
-                Dpr = 0.0;
-                Dpi = 0.0;
-            }
-            const double k1 = A * rcp64(Bm), k2 = C * rcp64(Bp);
-            const double nA = -Am * k1, nC = -Cp * k2;
-            const double nB = B - Cm * k1 - Ap * k2;
-            const double nDr = Dr - Dmr * k1 - Dpr * k2;
-            const double nDi = Di - Dmi * k1 - Dpi * k2;
-            A = nA;
-            B = nB;
-            C = nC;
-            Dr = nDr;
-            Di = nDi;
-        }
-        const double iB = rcp64(B);
-        const double Xr = Dr * iB, Xi = Di * iB;
-        const int sprev = (l > 0) ? l - 1 : l;
-        double xmr = shfl_grp<L>(Xr, sprev, base), xmi = shfl_grp<L>(Xi, sprev, base);
-        if (l == 0) {
-            xmr = 0.0;
-            xmi = 0.0;
-        }
-        // back substitution: x_i = delta_i - alpha_i x_{s-1} - gamma_i x_{i+1}
-        double xr = Xr, xi = Xi;
-        double xsr[RESID ? KMAX : 1], xsi[RESID ? KMAX : 1];
 #pragma unroll
-        for (int k = KMAX - 1; k >= 0; --k) {
-            if (k < cnt) {
-                if (k < cnt - 1) {
-                    xr = dr[k] - al[k] * xmr - ga[k] * xr;
-                    xi = di[k] - al[k] * xmi - ga[k] * xi;
+            for (int k = 0; k < KMAX; ++k) {
+                if (k < cnt) {
+                    double a, b, c;
+                    coef(k, a, b, c);
+                    const double pr = (k == 0) ? pmr : (double)xr[k > 0 ? k - 1 : 0];
+                    const double pi = (k == 0) ? pmi : (double)xi[k > 0 ? k - 1 : 0];
+                    const double qr = (k == cnt - 1) ? x0r : (double)xr[k + 1 < KMAX ? k + 1 : k];
+                    const double qi = (k == cnt - 1) ? x0i : (double)xi[k + 1 < KMAX ? k + 1 : k];
+                    const float2 d = zs(g, k, l);
+                    const double fre = (n == 0) ? 0.0 : (double)d.x, fim = (double)d.y;
+                    const double er = fre - (a * pr + b * (double)xr[k] + c * qr);
+                    const double ei = fim - (a * pi + b * (double)xi[k] + c * qi);
+                    use(k, er, ei, fre, fim);
                 }
+            }
+        };
+        T xr[KMAX], xi[KMAX];
+        // the staged right-hand side; mode 0 is dropped (its slot's real part)
+        solve(std::false_type{}, [&](int k) {
+            const float2 d = zs(g, k, l);
+            return make_float2((n == 0) ? 0.f : d.x, d.y);
+        }, xr, xi);
+        if constexpr (std::is_same<T, float>::value) {
+            // FP32 elimination: the low modes, the least diagonally dominant,
+            // get one pass of iterative refinement with the residual formed in
+            // FP64 (x += T^-1 r, the correction solved in FP32); uniform over
+            // the block.  The residual is kept in registers when the
+            // residual report still needs the staged right-hand side, else
+            // written over it.
+            if (tile.row0 < kTriFp64Modes) {
                 if constexpr (RESID) {
-                    xsr[k] = (n == 0) ? 0.0 : xr;
-                    xsi[k] = xi;
-                } else if (p64) {
-                    spec64[(size_t)n * nT + r0 + k] = make_double2((n == 0) ? 0.0 : xr, xi);
+                    float2 rk[KMAX];
+                    residual(xr, xi, [&](int k, double er, double ei, double, double) {
+                        rk[k] = make_float2((float)er, (float)ei);
+                    });
+                    solve(std::true_type{}, [&](int k) { return rk[k]; }, xr, xi);
                 } else {
-                    zs(g, k, l) = make_float2((n == 0) ? 0.f : (float)xr, (float)xi);
+                    float2 rk[KMAX];
+                    residual(xr, xi, [&](int k, double er, double ei, double, double) {
+                        rk[k] = make_float2((float)er, (float)ei);
+                    });
+#pragma unroll
+                    for (int k = 0; k < KMAX; ++k)
+                        if (k < cnt) zs(g, k, l) = rk[k];
+                    solve(std::true_type{}, [&](int k) { return zs(g, k, l); }, xr, xi);
                 }
             }
         }
         if constexpr (RESID) {
-            // neighbours across lane boundaries: the next lane's first row
-            const int snext = (l + 1 < L) ? l + 1 : l;
-            const double x0r = shfl_grp<L>(xsr[0], snext, base), x0i = shfl_grp<L>(xsi[0], snext, base);
-            double rr = 0.0, dd = 0.0;
-#pragma unroll
-            for (int k = 0; k < KMAX; ++k) {
-                if (k < cnt) {
-                    const int i = r0 + k;
-                    const float2 d = zs(g, k, l);
-                    const double dre = (n == 0) ? 0.0 : (double)d.x, dim = (double)d.y;
-                    const double ct = cs(k, l);
-                    double a = h2 - ct, c = h2 + ct;
-                    double b = -2.0 * h2 - nn * ss(k, l);
-                    if (i == 0) {
-                        b += a;
-                        a = 0.0;
-                    }
-                    if (i == nT - 1) {
-                        b += c;
-                        c = 0.0;
-                    }
-                    const double pr = (k == 0) ? xmr : xsr[k > 0 ? k - 1 : 0];
-                    const double pi = (k == 0) ? xmi : xsi[k > 0 ? k - 1 : 0];
-                    const double qr = (k == cnt - 1) ? x0r : xsr[k + 1 < KMAX ? k + 1 : k];
-                    const double qi = (k == cnt - 1) ? x0i : xsi[k + 1 < KMAX ? k + 1 : k];
-                    const double er = dre - (a * pr + b * xsr[k] + c * qr);
-                    const double ei = dim - (a * pi + b * xsi[k] + c * qi);
-                    rr += er * er + ei * ei;
-                    dd += dre * dre + dim * dim;
-                }
-            }
+            double rsum = 0.0, dsum = 0.0;
+            residual(xr, xi, [&](int, double er, double ei, double fre, double fim) {
+                rsum += er * er + ei * ei;
+                dsum += fre * fre + fim * fim;
+            });
             for (int m = 1; m < L; m <<= 1) {
-                rr += __shfl_xor(rr, m, 64);
-                dd += __shfl_xor(dd, m, 64);
+                rsum += __shfl_xor(rsum, m, 64);
+                dsum += __shfl_xor(dsum, m, 64);
             }
-            if (active && l == 0) s.resid[n] = dd > 0.0 ? sqrt(rr / dd) : 0.0;
+            if (active && l == 0) s.resid[n] = dsum > 0.0 ? sqrt(rsum / dsum) : 0.0;
+        }
 #pragma unroll
-            for (int k = 0; k < KMAX; ++k)
-                if (k < cnt) {
-                    if (p64)
-                        spec64[(size_t)n * nT + r0 + k] = make_double2(xsr[k], xsi[k]);
-                    else
-                        zs(g, k, l) = make_float2((float)xsr[k], (float)xsi[k]);
-                }
+        for (int k = 0; k < KMAX; ++k) {
+            if (k < cnt) {
+                if (p64)
+                    spec64[(size_t)n * nT + r0 + k] = make_double2((n == 0) ? 0.0 : (double)xr[k], (double)xi[k]);
+                else
+                    zs(g, k, l) = make_float2((n == 0) ? 0.f : (float)xr[k], (float)xi[k]);
+            }
         }
         __syncthreads();
         if (p64) return;  // the solution is already stored (uniform over the block)
@@ -1383,19 +1413,19 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
 }
 
 
-template <int L, int KMAX, bool RESID>
+template <int L, int KMAX, bool RESID, typename T>
 __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sims,
                                                  const Tile* __restrict__ tiles, int ntiles, int parity) {
     __shared__ float2 zs[(256 / L) * KMAX * (L + 1)];
     __shared__ double cs[KMAX * (L + 1)], ss[KMAX * (L + 1)];
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles)
-        tri_tile<L, KMAX, RESID>(sims, tiles[t], parity, zs, cs, ss);
+        tri_tile<L, KMAX, RESID, T>(sims, tiles[t], parity, zs, cs, ss);
 }
 
 // Every class with K = 8 rows per lane in one launch: a tile carries its lanes
 // per mode in col0, and the block takes that variant (mixed-size batches
 // would otherwise queue one small launch per class).
-template <bool RESID>
+template <bool RESID, typename T>
 __global__ __launch_bounds__(256) void k_tridiag_mixed(const SimDesc* __restrict__ sims,
                                                        const Tile* __restrict__ tiles, int ntiles,
                                                        int parity) {
@@ -1405,10 +1435,10 @@ __global__ __launch_bounds__(256) void k_tridiag_mixed(const SimDesc* __restrict
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
         switch (tile.col0) {
-            case 8: tri_tile<8, K, RESID>(sims, tile, parity, zs, cs, ss); break;
-            case 16: tri_tile<16, K, RESID>(sims, tile, parity, zs, cs, ss); break;
-            case 32: tri_tile<32, K, RESID>(sims, tile, parity, zs, cs, ss); break;
-            default: tri_tile<64, K, RESID>(sims, tile, parity, zs, cs, ss); break;
+            case 8: tri_tile<8, K, RESID, T>(sims, tile, parity, zs, cs, ss); break;
+            case 16: tri_tile<16, K, RESID, T>(sims, tile, parity, zs, cs, ss); break;
+            case 32: tri_tile<32, K, RESID, T>(sims, tile, parity, zs, cs, ss); break;
+            default: tri_tile<64, K, RESID, T>(sims, tile, parity, zs, cs, ss); break;
         }
     }
 }
@@ -1607,24 +1637,24 @@ hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, in
     return hipGetLastError();
 }
 
-hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles,
-                          int ntiles, int parity, int gridCap, hipStream_t st, bool resid) {
-    if (ntiles <= 0) return hipSuccess;
-    const dim3 grid(grid_for(ntiles, gridCap)), block(256);
+template <typename T>
+static hipError_t launch_tridiag_t(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles,
+                                   int ntiles, int parity, dim3 grid, hipStream_t st, bool resid) {
+    const dim3 block(256);
     if (lanesPerMode == 0 && kmax == 8) {
         if (resid)
-            hipLaunchKernelGGL((k_tridiag_mixed<true>), grid, block, 0, st, sims, tiles, ntiles, parity);
+            hipLaunchKernelGGL((k_tridiag_mixed<true, T>), grid, block, 0, st, sims, tiles, ntiles, parity);
         else
-            hipLaunchKernelGGL((k_tridiag_mixed<false>), grid, block, 0, st, sims, tiles, ntiles, parity);
+            hipLaunchKernelGGL((k_tridiag_mixed<false, T>), grid, block, 0, st, sims, tiles, ntiles, parity);
         return hipGetLastError();
     }
 #define KAM_TRI(LL, KK)                                                                        \
     if (lanesPerMode == LL && kmax == KK) {                                                    \
         if (resid)                                                                             \
-            hipLaunchKernelGGL((k_tridiag<LL, KK, true>), grid, block, 0, st, sims, tiles, ntiles, \
+            hipLaunchKernelGGL((k_tridiag<LL, KK, true, T>), grid, block, 0, st, sims, tiles, ntiles, \
                                parity);                                                        \
         else                                                                                   \
-            hipLaunchKernelGGL((k_tridiag<LL, KK, false>), grid, block, 0, st, sims, tiles, ntiles, \
+            hipLaunchKernelGGL((k_tridiag<LL, KK, false, T>), grid, block, 0, st, sims, tiles, ntiles, \
                                parity);                                                        \
         return hipGetLastError();                                                              \
     }
@@ -1636,6 +1666,14 @@ hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const
     KAM_TRI(64, 32)
 #undef KAM_TRI
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles, int ntiles,
+                          int parity, int gridCap, hipStream_t st, bool resid, bool fp32) {
+    if (ntiles <= 0) return hipSuccess;
+    const dim3 grid(grid_for(ntiles, gridCap));
+    return fp32 ? launch_tridiag_t<float>(lanesPerMode, kmax, sims, tiles, ntiles, parity, grid, st, resid)
+                : launch_tridiag_t<double>(lanesPerMode, kmax, sims, tiles, ntiles, parity, grid, st, resid);
 }
 
 hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,

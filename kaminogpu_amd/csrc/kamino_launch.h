@@ -21,14 +21,32 @@ hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, in
                           size_t ldsBytes, int gridCap, hipStream_t st);
 // `parity` names the current state copy: a simulation with the FP64 pressure
 // keeps its solved spectrum over the other one.
+// fp32: run the elimination in FP32 (the FP32 pressure; tiles holding modes
+// below kTriFp64Modes then refine their solution once with an FP64
+// residual); else in FP64 (the FP64 pressure).
 hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles,
-                          int ntiles, int parity, int gridCap, hipStream_t st, bool resid = false);
+                          int ntiles, int parity, int gridCap, hipStream_t st, bool resid = false,
+                          bool fp32 = false);
+constexpr int kTriFp64Modes = 16;
 // p64: the tiles belong to simulations with the FP64 pressure (SIM_P64).
 // pout (one simulation only): write its pressure rows there instead of
 // updating the velocities.
 hipError_t launch_ifft_grad(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
                             size_t ldsBytes, int gridCap, hipStream_t st, bool p64 = false,
                             float* pout = nullptr);
+
+// The phi transforms of SIM_DFT simulations (kamino_dft.hip): divergence rows
+// + forward DFT product, and inverse DFT product + gradient rows.  rowTiles
+// are row runs of the simulations, gemmTiles 64-row x 128-column output
+// blocks (row0, rows, col0).
+hipError_t launch_dft_forward(const SimDesc* sims, const Tile* rowTiles, int nRow, const Tile* gemmTiles,
+                              int nGemm, int parity, hipStream_t st);
+hipError_t launch_dft_inverse(const SimDesc* sims, const Tile* rowTiles, int nRow, const Tile* gemmTiles,
+                              int nGemm, int parity, hipStream_t st);
+// The inverse DFT product alone: pressure rows over the finished state copy.
+hipError_t launch_dft_pressure(const SimDesc* sims, const Tile* gemmTiles, int nGemm, int parity,
+                               hipStream_t st);
+constexpr int kDftTileRows = 64, kDftTileCols = 128, kDftChunk = 32;
 
 // Raises the dynamic-LDS limit of every step kernel to `bytes` (call once).
 hipError_t configure_kernels(size_t bytes);

@@ -168,6 +168,7 @@ extern "C" int kam_sim_step_noalloc(const kam_sim_desc* d, float* u_phi, float* 
     if (rc != KAM_OK) return rc;
     rc = check_precision(d->precision, plan);
     if (rc != KAM_OK) return rc;
+    plan.transform = KAM_TRANSFORM_FFT;  // the DFT's matrices would need an allocation
     rc = ensure_device(d->device);  // makes the device current
     if (rc != KAM_OK) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);

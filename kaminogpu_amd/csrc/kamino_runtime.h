@@ -35,6 +35,7 @@ struct Plan {
     int triK = 0;      // template row capacity per lane
     size_t advLds = 0; // bytes of LDS of an advection tile (geometric on)
     size_t fftLds = 0; // bytes of LDS of an FFT tile (forward or inverse)
+    int transform = 0; // KAM_TRANSFORM_FFT or KAM_TRANSFORM_DFT (FP32-pressure simulations)
 };
 
 // Validates a shape and fills its plan; returns KAM_OK or KAM_EINVAL.
@@ -45,6 +46,13 @@ int check_precision(int precision, const Plan& plan);
 int adjust_plan(Plan& plan, int advRows, int fftRows);
 // Records tile shapes for n_theta (0 = default); later plans use them.
 int tune_set(int nTheta, int advRows, int advCols, int fftRows);
+// Records the phi transform for n_theta (KAM_TRANSFORM_*; -1 = default FFT).
+int tune_set_transform(int nTheta, int transform);
+
+// Sets or clears SIM_DFT on a simulation's descriptor from its plan and
+// precision, acquiring the shape's DFT matrices when needed (cached per
+// device and nPhi for the life of the process).
+int attach_transform(SimDesc& d, const Plan& p, int precision, int device);
 
 // Device tables shared by every simulation of one (device, shape, dt, R).
 struct Tables {
@@ -72,6 +80,7 @@ size_t sim_floats(int nTheta, int nPhi);
 // The launch sequence of one step over a set of simulations.
 struct TriClass {
     int L, K, off, count;
+    bool fp32;  // FP32 elimination (see launch_tridiag)
 };
 
 // Tile lists of one step over a set of simulations, as ranges of one array.
@@ -80,8 +89,9 @@ struct TileLayout {
     int advOff = 0, nAdv = 0;
     int fftOff = 0, nFft = 0, nFft32 = 0, nFft64 = 0;  // FFT tiles (FP32-pressure ones first)
     int partOff = 0, nPart = 0;
+    int dftRowOff = 0, nDftRow = 0, dftOff = 0, nDft = 0;  // SIM_DFT rows and MFMA product tiles
     int ntri = 0;
-    TriClass tri[6] = {};
+    TriClass tri[12] = {};
     size_t advLds = 0, fftLds = 0, fftLds64 = 0;
     int advStride = 0;  // staged row stride of the advection launch
 };
@@ -114,7 +124,8 @@ public:
     // (`parity`: the current state copy).
     int launch_tri_residual(hipStream_t st, int parity);
     // Queues the inverse transform of the solved spectrum of a one-simulation
-    // batch, writing its pressure to pout (device, float [nTheta][nPhi]).
+    // batch, writing its pressure to pout (device, float [nTheta][nPhi]); a
+    // SIM_DFT simulation writes it over its finished state copy instead.
     int launch_pressure(hipStream_t st, int parity, float* pout);
     int nsims() const { return lay_.nsims; }
     size_t capacityBytes() const { return capSims_ * sizeof(SimDesc) + capTiles_ * sizeof(Tile); }

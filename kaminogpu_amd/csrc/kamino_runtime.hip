@@ -84,13 +84,23 @@ static constexpr size_t kLdsMax = 160 * 1024;
 // ---------------------------------------------------------------------------
 struct TuneEntry {
     int advRows = 0, advCols = 0, fftRows = 0;
+    int transform = -1;  // KAM_TRANSFORM_*, -1 = default (FFT)
 };
 static std::mutex g_tune_mu;
 static std::map<int, TuneEntry> g_tune;
 
 int tune_set(int nT, int advRows, int advCols, int fftRows) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[nT] = TuneEntry{advRows, advCols, fftRows};
+    TuneEntry& e = g_tune[nT];
+    e.advRows = advRows;
+    e.advCols = advCols;
+    e.fftRows = fftRows;
+    return KAM_OK;
+}
+
+int tune_set_transform(int nT, int transform) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[nT].transform = transform;
     return KAM_OK;
 }
 
@@ -135,6 +145,8 @@ int make_plan(int nT, int nP, Plan& p) {
     else { p.triL = 64; p.triK = 32; }
     // tuned shapes, then manual overrides for tile sweeps
     const TuneEntry te = tune_get(nT);
+    // the MFMA DFT holds nPhi x nPhi matrices: offered up to n_theta = 1024
+    if (te.transform == KAM_TRANSFORM_DFT && nT <= 1024) p.transform = KAM_TRANSFORM_DFT;
     if (te.advCols >= 8 && te.advCols <= kAdvMaxCols) p.advCols = te.advCols < nP ? te.advCols : nP;
     if (te.advRows > 0 || te.fftRows > 0) adjust_plan(p, te.advRows, te.fftRows);
     if (const char* v = std::getenv("KAM_ADV_COLS")) {
@@ -252,6 +264,114 @@ void release_tables(int device, int nT, double dt, double radius) {
     (void)hipFree(it->second.t.twN);
     (void)hipFree(it->second.t.twN64);
     g_tabs.erase(it);
+}
+
+// ---------------------------------------------------------------------------
+// DFT matrices of the MFMA phi transform (kamino_dft.hip), per device and
+// nPhi, built once and kept for the life of the process: FP16 hi and lo
+// planes of 2^13 W, [N][K] with N = nPhi rounded up to 128 output columns and
+// K = nPhi rounded up to 32 (padding zero).
+//   forward  W[2n][i] = cos(2 pi n i / nPhi), W[2n+1][i] = -sin(...), n < M;
+//            W[1][i] = (-1)^i (the Nyquist mode packed into slot 0)
+//   inverse  W[i][2n] = 2 cos(2 pi n i / nPhi), W[i][2n+1] = -2 sin(...), 0 < n < M;
+//            W[i][0] = 0 (mode 0 dropped), W[i][1] = (-1)^i
+// ---------------------------------------------------------------------------
+static uint16_t f16_bits(double x) {  // round to nearest even, normal and subnormal range
+    const float f = (float)x;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const double a = std::fabs(x);
+    if (a == 0.0) return (uint16_t)sign;
+    int e;
+    (void)std::frexp(a, &e);  // a = m 2^e, m in [0.5, 1)
+    const int ex = std::max(e - 1, -14);  // exponent of the binade (subnormals share -14)
+    const double q = std::ldexp(a, 10 - ex);  // significand in units of the last place
+    double r = std::nearbyint(q);            // ties to even under the default rounding mode
+    int exOut = ex;
+    if (r >= 2048.0) {  // rounded up into the next binade
+        r /= 2.0;
+        ++exOut;
+    }
+    if (exOut > 15) return (uint16_t)(sign | 0x7c00u);
+    const uint32_t mant = (uint32_t)r;
+    if (mant < 1024) return (uint16_t)(sign | mant);  // subnormal (exOut = -14)
+    return (uint16_t)(sign | ((uint32_t)(exOut + 15) << 10) | (mant - 1024));
+}
+static double f16_value(uint16_t h) {
+    const int e = (h >> 10) & 0x1f, m = h & 0x3ff;
+    const double v = (e == 0) ? std::ldexp((double)m, -24) : std::ldexp(1024.0 + m, e - 25);
+    return (h & 0x8000u) ? -v : v;
+}
+
+struct DftTables {
+    uint16_t* fwd[2] = {nullptr, nullptr};
+    uint16_t* inv[2] = {nullptr, nullptr};
+    int K = 0;
+};
+static std::mutex g_dft_mu;
+static std::map<std::pair<int, int>, DftTables> g_dft;
+
+static int acquire_dft(int device, int nP, DftTables* out) {
+    std::lock_guard<std::mutex> lk(g_dft_mu);
+    auto it = g_dft.find({device, nP});
+    if (it != g_dft.end()) {
+        *out = it->second;
+        return KAM_OK;
+    }
+    const int M = nP / 2;
+    const int N = (nP + kDftTileCols - 1) / kDftTileCols * kDftTileCols;
+    const int K = (nP + kDftChunk - 1) / kDftChunk * kDftChunk;
+    std::vector<uint16_t> fh((size_t)N * K, 0), fl((size_t)N * K, 0), ih((size_t)N * K, 0), il((size_t)N * K, 0);
+    const double scale = 8192.0;
+    auto put = [&](std::vector<uint16_t>& hi, std::vector<uint16_t>& lo, int c, int k, double w) {
+        const double x = w * scale;
+        const uint16_t h = f16_bits(x);
+        hi[(size_t)c * K + k] = h;
+        lo[(size_t)c * K + k] = f16_bits(x - f16_value(h));
+    };
+    for (int i = 0; i < nP; ++i) {
+        const double alt = (i & 1) ? -1.0 : 1.0;
+        for (int n = 0; n < M; ++n) {
+            // the angle 2 pi n i / nPhi reduced exactly: n i mod nPhi
+            const double a = 2.0 * kPi * (double)(((long long)n * i) % nP) / nP;
+            const double cs = std::cos(a), sn = std::sin(a);
+            put(fh, fl, 2 * n, i, cs);
+            put(fh, fl, 2 * n + 1, i, n == 0 ? alt : -sn);
+            put(ih, il, i, 2 * n, n == 0 ? 0.0 : 2.0 * cs);
+            put(ih, il, i, 2 * n + 1, n == 0 ? alt : -2.0 * sn);
+        }
+    }
+    KAM_HIP(hipSetDevice(device));
+    DftTables t;
+    t.K = K;
+    const size_t bytes = (size_t)N * K * sizeof(uint16_t);
+    std::vector<uint16_t>* src[4] = {&fh, &fl, &ih, &il};
+    uint16_t** dst[4] = {&t.fwd[0], &t.fwd[1], &t.inv[0], &t.inv[1]};
+    for (int q = 0; q < 4; ++q) {
+        if (hipMalloc(dst[q], bytes) != hipSuccess) return set_err(KAM_ENOMEM, "DFT matrix allocation failed");
+        KAM_HIP(hipMemcpy(*dst[q], src[q]->data(), bytes, hipMemcpyHostToDevice));
+    }
+    g_dft[{device, nP}] = t;
+    *out = t;
+    return KAM_OK;
+}
+
+int attach_transform(SimDesc& d, const Plan& p, int precision, int device) {
+    d.flags &= ~SIM_DFT;
+    d.dftFwd[0] = d.dftFwd[1] = d.dftInv[0] = d.dftInv[1] = nullptr;
+    d.dftK = 0;
+    if (p.transform != KAM_TRANSFORM_DFT || precision != KAM_PREC_FP32) return KAM_OK;
+    DftTables t;
+    int rc = acquire_dft(device, p.nP, &t);
+    if (rc != KAM_OK) return rc;
+    d.flags |= SIM_DFT;
+    d.dftFwd[0] = t.fwd[0];
+    d.dftFwd[1] = t.fwd[1];
+    d.dftInv[0] = t.inv[0];
+    d.dftInv[1] = t.inv[1];
+    d.dftK = t.K;
+    return KAM_OK;
 }
 
 void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& sd, float* mem) {
@@ -428,7 +548,7 @@ int plan_tiles(SimDesc* sims, const Plan* plans, int n, int* rowScratch, Tile* o
         for (int s = 0; s < n; ++s) {
             const Plan& p = plans[s];
             const bool p64 = (sims[s].flags & SIM_P64) != 0;
-            if (p64 != (pass == 1)) continue;
+            if (p64 != (pass == 1) || (sims[s].flags & SIM_DFT)) continue;
             int R = fftRows[s];
             if (p64)
                 while (R > 1 && fft_lds(R, p.M, true) > kLdsMax) --R;
@@ -441,6 +561,20 @@ int plan_tiles(SimDesc* sims, const Plan* plans, int n, int* rowScratch, Tile* o
         }
     L.nFft = cnt - L.fftOff;
     L.nFft32 = L.nFft - L.nFft64;
+    // SIM_DFT simulations: row runs for the divergence and gradient kernels,
+    // then 64-row x 128-column blocks of the DFT products
+    L.dftRowOff = cnt;
+    for (int s = 0; s < n; ++s)
+        if (sims[s].flags & SIM_DFT)
+            for (int j = 0; j < plans[s].nT; j += fftRows[s]) put(Tile{s, j, std::min(fftRows[s], plans[s].nT - j), 0});
+    L.nDftRow = cnt - L.dftRowOff;
+    L.dftOff = cnt;
+    for (int s = 0; s < n; ++s)
+        if (sims[s].flags & SIM_DFT)
+            for (int j = 0; j < plans[s].nT; j += kDftTileRows)
+                for (int c = 0; c < plans[s].nP; c += kDftTileCols)
+                    put(Tile{s, j, std::min(kDftTileRows, plans[s].nT - j), c});
+    L.nDft = cnt - L.dftOff;
     // tridiagonal tiles, grouped by kernel class; when several classes of
     // K = 8 rows per lane are present they share one mixed launch (L = 0),
     // each tile carrying its lanes per mode in col0
@@ -454,22 +588,29 @@ int plan_tiles(SimDesc* sims, const Plan* plans, int n, int* rowScratch, Tile* o
                     break;
                 }
     const bool mixed = k8 > 1;
-    for (auto& c : classes) {
-        const bool inMixed = mixed && c[1] == 8;
-        const int off = cnt;
-        for (int s = 0; s < n; ++s) {
-            const Plan& p = plans[s];
-            if (p.triL != c[0] || p.triK != c[1]) continue;
-            const int per = 256 / p.triL;
-            for (int m = 0; m < p.M; m += per) put(Tile{s, m, std::min(per, p.M - m), inMixed ? p.triL : 0});
+    // FP32-pressure simulations run the elimination in FP32 (the low-mode
+    // tiles refining once in FP64); FP64-pressure ones in FP64
+    for (int fp32 = 0; fp32 < 2; ++fp32)
+        for (auto& c : classes) {
+            const bool inMixed = mixed && c[1] == 8;
+            const int off = cnt;
+            for (int s = 0; s < n; ++s) {
+                const Plan& p = plans[s];
+                if (p.triL != c[0] || p.triK != c[1]) continue;
+                const bool p64 = (sims[s].flags & SIM_P64) != 0;
+                const int per = 256 / p.triL;
+                for (int m = 0; m < p.M; m += per) {
+                    if (p64 == (fp32 == 1)) continue;
+                    put(Tile{s, m, std::min(per, p.M - m), inMixed ? p.triL : 0});
+                }
+            }
+            const int num = cnt - off;
+            if (num == 0) continue;
+            if (inMixed && L.ntri > 0 && L.tri[L.ntri - 1].L == 0 && L.tri[L.ntri - 1].fp32 == (fp32 == 1))
+                L.tri[L.ntri - 1].count += num;  // classes are contiguous in the tile list
+            else
+                L.tri[L.ntri++] = TriClass{inMixed ? 0 : c[0], c[1], off, num, fp32 == 1};
         }
-        const int num = cnt - off;
-        if (num == 0) continue;
-        if (inMixed && L.ntri > 0 && L.tri[L.ntri - 1].L == 0)
-            L.tri[L.ntri - 1].count += num;  // classes are contiguous in the tile list
-        else
-            L.tri[L.ntri++] = TriClass{inMixed ? 0 : c[0], c[1], off, num};
-    }
     // particle tiles: runs of kPartTile particles
     constexpr int kPartTile = 2048;
     L.partOff = cnt;
@@ -490,7 +631,7 @@ int launch_tri(const SimDesc* dSims, const Tile* dTiles, const TileLayout& L, in
     // membership changes)
     for (int k = 0; k < L.ntri; ++k) {
         const TriClass& c = L.tri[k];
-        KAM_HIP(launch_tridiag(c.L, c.K, dSims, dTiles + c.off, c.count, parity, kGridCap, st, resid));
+        KAM_HIP(launch_tridiag(c.L, c.K, dSims, dTiles + c.off, c.count, parity, kGridCap, st, resid, c.fp32));
     }
     return KAM_OK;
 }
@@ -512,6 +653,7 @@ int launch_step(const SimDesc* dSims, const Tile* dTiles, const TileLayout& L, i
     if (phases & KAM_PHASE_DIVFFT) {
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT], st));
         KAM_HIP(launch_div_fft(dSims, dTiles + L.fftOff, L.nFft, p, L.fftLds, kGridCap, st));
+        KAM_HIP(launch_dft_forward(dSims, dTiles + L.dftRowOff, L.nDftRow, dTiles + L.dftOff, L.nDft, p, st));
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT + 1], st));
     }
     if (phases & KAM_PHASE_TRI) {
@@ -525,6 +667,7 @@ int launch_step(const SimDesc* dSims, const Tile* dTiles, const TileLayout& L, i
         KAM_HIP(launch_ifft_grad(dSims, dTiles + L.fftOff, L.nFft32, p, L.fftLds, kGridCap, st, false));
         KAM_HIP(launch_ifft_grad(dSims, dTiles + L.fftOff + L.nFft32, L.nFft64, p, L.fftLds64, kGridCap, st,
                                  true));
+        KAM_HIP(launch_dft_inverse(dSims, dTiles + L.dftRowOff, L.nDftRow, dTiles + L.dftOff, L.nDft, p, st));
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_IFFT + 1], st));
     }
     *newParity = p;
@@ -565,6 +708,7 @@ int DeviceBatch::launch_pressure(hipStream_t st, int parity, float* pout) {
     KAM_HIP(launch_ifft_grad(dSims_, dTiles_ + L.fftOff, L.nFft32, parity, L.fftLds, kGridCap, st, false, pout));
     KAM_HIP(launch_ifft_grad(dSims_, dTiles_ + L.fftOff + L.nFft32, L.nFft64, parity, L.fftLds64, kGridCap, st,
                              true, pout));
+    KAM_HIP(launch_dft_pressure(dSims_, dTiles_ + L.dftOff, L.nDft, parity, st));
     return KAM_OK;
 }
 
@@ -685,6 +829,11 @@ int kam_sim_create(const kam_sim_desc* d, kam_sim** out) {
         return set_err(KAM_EHIP, "stream creation / clear failed");
     }
     fill_desc(s->desc, plan, s->tables, *d, s->mem);
+    rc = attach_transform(s->desc, plan, d->precision, d->device);
+    if (rc != KAM_OK) {
+        kam_sim_destroy(s);
+        return rc;
+    }
     std::vector<SimDesc> sims{s->desc};
     std::vector<Plan> plans{plan};
     rc = s->batch.build(sims, plans, s->stream);
@@ -830,7 +979,9 @@ int kam_sim_get_pressure(kam_sim* s, float* p) {
     }
     int rc = s->batch.launch_pressure(s->stream, s->parity, s->pbuf);
     if (rc != KAM_OK) return rc;
-    KAM_HIP(hipMemcpyAsync(p, s->pbuf, N * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    // a DFT simulation leaves its pressure rows over the finished state copy
+    const float* src = (s->desc.flags & SIM_DFT) ? s->desc.state[s->parity ^ 1][F_UTHETA] : s->pbuf;
+    KAM_HIP(hipMemcpyAsync(p, src, N * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     KAM_HIP(hipStreamSynchronize(s->stream));
     return KAM_OK;
 }
