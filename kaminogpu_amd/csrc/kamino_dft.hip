@@ -69,13 +69,24 @@ __global__ __launch_bounds__(256) void k_dft_gemm(const SimDesc* __restrict__ si
         const _Float16* __restrict__ gHi = reinterpret_cast<const _Float16*>(INV ? s.dftInv[0] : s.dftFwd[0]);
         const _Float16* __restrict__ gLo = reinterpret_cast<const _Float16*>(INV ? s.dftInv[1] : s.dftFwd[1]);
 
-        // per-row scale: max |A[j][.]| over the whole row, four partial maxima
-        // per row (threads of equal tid % 64 share a row)
-        {
+        // per-row scale: max |A[j][.]| over the whole row.  Forward: each
+        // wavefront walks rows with its lanes along the row (contiguous
+        // reads), then reduces across lanes; inverse (mode-major spectrum):
+        // lanes along the rows, four partial maxima per row.
+        if constexpr (INV) {
             const int r = tid & 63, q = tid >> 6;
             unsigned m = 0u;
             for (int k = q; k < nP; k += 4) m = max(m, __float_as_uint(load_a<INV>(s, a, j0 + r, k)) & 0x7fffffffu);
             rowMax[q][r] = m;
+        } else {
+            for (int r = wave; r < kDftRows; r += 4) {
+                unsigned m = 0u;
+                for (int k = lane; k < nP; k += 64)
+                    m = max(m, __float_as_uint(load_a<INV>(s, a, j0 + r, k)) & 0x7fffffffu);
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, o, 64));
+                if (lane < 4) rowMax[lane][r] = m;
+            }
         }
         __syncthreads();
         if (tid < kDftRows) {
@@ -90,37 +101,45 @@ __global__ __launch_bounds__(256) void k_dft_gemm(const SimDesc* __restrict__ si
         floatx16 acc[2];
         for (int b = 0; b < 2; ++b)
             for (int v = 0; v < 16; ++v) acc[b][v] = 0.f;
-        for (int k0 = 0; k0 < K; k0 += kDftK) {
-            // stage A: 64 rows x 32 k, thread -> row tid / 4, k run (tid % 4) * 8
-            {
-                const int r = tid >> 2, kq = (tid & 3) * 8;
-                const int ex = rowExp[r];
-                float x[8];
+        // software pipeline: the next chunk's A values and B halves are loaded
+        // into registers while the MFMAs run on the current chunk in LDS.
+        // A: 64 rows x 32 k, thread -> row tid / 4, k run (tid % 4) * 8;
+        // B: 128 columns x 32 k, thread -> column tid / 2, k run (tid % 2) * 16
+        const int ar = tid >> 2, akq = (tid & 3) * 8, bc = tid >> 1, bkh = (tid & 1) * 16;
+        float xa[8];
+        half8 xb[4];
+        auto fetch = [&](int k0) {
 #pragma unroll
-                for (int u = 0; u < 8; ++u) x[u] = ldexpf(load_a<INV>(s, a, j0 + r, k0 + kq + u), ex);
+            for (int u = 0; u < 8; ++u) xa[u] = load_a<INV>(s, a, j0 + ar, k0 + akq + u);
+            const size_t g = (size_t)(c0 + bc) * K + k0 + bkh;
+            const half8* sh = reinterpret_cast<const half8*>(gHi + g);
+            const half8* sl = reinterpret_cast<const half8*>(gLo + g);
+            xb[0] = sh[0];
+            xb[1] = sh[1];
+            xb[2] = sl[0];
+            xb[3] = sl[1];
+        };
+        fetch(0);
+        for (int k0 = 0; k0 < K; k0 += kDftK) {
+            {
+                const int ex = rowExp[ar];
                 half8 h, l;
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     _Float16 hh, ll;
-                    split_h(x[u], hh, ll);
+                    split_h(ldexpf(xa[u], ex), hh, ll);
                     h[u] = hh;
                     l[u] = ll;
                 }
-                *reinterpret_cast<half8*>(aHi + r * kAPad + kq) = h;
-                *reinterpret_cast<half8*>(aLo + r * kAPad + kq) = l;
-            }
-            // stage B: 128 columns x 32 k, thread -> column tid / 2, k run (tid % 2) * 16
-            {
-                const int c = tid >> 1, kh = (tid & 1) * 16;
-                const size_t g = (size_t)(c0 + c) * K + k0 + kh;
-                const half8* sh = reinterpret_cast<const half8*>(gHi + g);
-                const half8* sl = reinterpret_cast<const half8*>(gLo + g);
-                *reinterpret_cast<half8*>(bHi + c * kAPad + kh) = sh[0];
-                *reinterpret_cast<half8*>(bHi + c * kAPad + kh + 8) = sh[1];
-                *reinterpret_cast<half8*>(bLo + c * kAPad + kh) = sl[0];
-                *reinterpret_cast<half8*>(bLo + c * kAPad + kh + 8) = sl[1];
+                *reinterpret_cast<half8*>(aHi + ar * kAPad + akq) = h;
+                *reinterpret_cast<half8*>(aLo + ar * kAPad + akq) = l;
+                *reinterpret_cast<half8*>(bHi + bc * kAPad + bkh) = xb[0];
+                *reinterpret_cast<half8*>(bHi + bc * kAPad + bkh + 8) = xb[1];
+                *reinterpret_cast<half8*>(bLo + bc * kAPad + bkh) = xb[2];
+                *reinterpret_cast<half8*>(bLo + bc * kAPad + bkh + 8) = xb[3];
             }
             __syncthreads();
+            if (k0 + kDftK < K) fetch(k0 + kDftK);
 #pragma unroll
             for (int ks = 0; ks < kDftK; ks += 16) {
                 const int kk = ks + 8 * (lane >> 5), cc = wave * 32 + (lane & 31);
