@@ -135,10 +135,13 @@ private:
     SimDesc* dSims_ = nullptr;
     Tile* dTiles_ = nullptr;
     size_t capSims_ = 0, capTiles_ = 0;
-    // pinned staging of the last upload; `uploaded_` marks when the copy is done
-    void* hStage_ = nullptr;
-    size_t stageBytes_ = 0;
-    hipEvent_t uploaded_ = nullptr;
+    // two pinned staging buffers used in turn; `uploaded_[b]` marks when the
+    // copy out of buffer b is done, so a rebuild waits only for the upload
+    // two rebuilds back (long done, normally) instead of the last one
+    void* hStage_[2] = {nullptr, nullptr};
+    size_t stageBytes_[2] = {0, 0};
+    hipEvent_t uploaded_[2] = {nullptr, nullptr};
+    int cur_ = 0;
     std::vector<Tile> hTiles_;
     TileLayout lay_;
 };

@@ -424,12 +424,14 @@ void set_tile_cols(SimDesc& d, int W) {
 // Device batches
 // ---------------------------------------------------------------------------
 DeviceBatch::~DeviceBatch() {
-    if (uploaded_) (void)hipEventSynchronize(uploaded_);
-
+    for (int b = 0; b < 2; ++b)
+        if (uploaded_[b]) (void)hipEventSynchronize(uploaded_[b]);
     if (dSims_) (void)hipFree(dSims_);
     if (dTiles_) (void)hipFree(dTiles_);
-    if (hStage_) (void)hipHostFree(hStage_);
-    if (uploaded_) (void)hipEventDestroy(uploaded_);
+    for (int b = 0; b < 2; ++b) {
+        if (hStage_[b]) (void)hipHostFree(hStage_[b]);
+        if (uploaded_[b]) (void)hipEventDestroy(uploaded_[b]);
+    }
 }
 
 // Grows the device tables geometrically; a regrowth frees the old tables
@@ -451,14 +453,15 @@ int DeviceBatch::reserve(size_t ns, size_t nt, hipStream_t st) {
         capTiles_ = c;
     }
     const size_t need = capSims_ * sizeof(SimDesc) + capTiles_ * sizeof(Tile);
-    if (need > stageBytes_) {
-        if (uploaded_) KAM_HIP(hipEventSynchronize(uploaded_));
-        if (hStage_) KAM_HIP(hipHostFree(hStage_));
-        hStage_ = nullptr;
-        KAM_HIP(hipHostMalloc(&hStage_, need, hipHostMallocDefault));
-        stageBytes_ = need;
+    const int b = cur_;
+    if (need > stageBytes_[b]) {
+        if (uploaded_[b]) KAM_HIP(hipEventSynchronize(uploaded_[b]));
+        if (hStage_[b]) KAM_HIP(hipHostFree(hStage_[b]));
+        hStage_[b] = nullptr;
+        KAM_HIP(hipHostMalloc(&hStage_[b], need, hipHostMallocDefault));
+        stageBytes_[b] = need;
     }
-    if (!uploaded_) KAM_HIP(hipEventCreateWithFlags(&uploaded_, hipEventDisableTiming));
+    if (!uploaded_[b]) KAM_HIP(hipEventCreateWithFlags(&uploaded_[b], hipEventDisableTiming));
     return KAM_OK;
 }
 
@@ -682,12 +685,13 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
     const int need = plan_tiles(sims.data(), plans.data(), n, rows.data(), nullptr, 0, lay_);
     hTiles_.resize((size_t)need);
     plan_tiles(sims.data(), plans.data(), n, rows.data(), hTiles_.data(), need, lay_);
+    cur_ ^= 1;
     int rc = reserve(sims.size(), hTiles_.size(), st);
     if (rc != KAM_OK) return rc;
-    // the staging buffer is free once the previous upload has landed (long
-    // ago, normally); the copies are queued behind the steps already launched
-    KAM_HIP(hipEventSynchronize(uploaded_));
-    char* stage = static_cast<char*>(hStage_);
+    // this staging buffer is free once its last upload, two rebuilds back,
+    // has landed; the copies are queued behind the steps already launched
+    KAM_HIP(hipEventSynchronize(uploaded_[cur_]));
+    char* stage = static_cast<char*>(hStage_[cur_]);
     const size_t sb = sims.size() * sizeof(SimDesc), tb = hTiles_.size() * sizeof(Tile);
     std::memcpy(stage, sims.data(), sb);
     std::memcpy(stage + capSims_ * sizeof(SimDesc), hTiles_.data(), tb);
@@ -695,7 +699,7 @@ int DeviceBatch::build(const std::vector<SimDesc>& simsIn, const std::vector<Pla
     if (tb)
         KAM_HIP(hipMemcpyAsync(dTiles_, stage + capSims_ * sizeof(SimDesc), tb,
                                hipMemcpyHostToDevice, st));
-    KAM_HIP(hipEventRecord(uploaded_, st));
+    KAM_HIP(hipEventRecord(uploaded_[cur_], st));
     return KAM_OK;
 }
 

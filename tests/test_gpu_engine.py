@@ -347,3 +347,39 @@ def test_cancel_of_a_finished_request_releases_it():
         got = eng.fetch(c, 48)
         for g, x in zip(got, alone(48, r["dt"], r["state"], 2)):
             np.testing.assert_array_equal(g, x)
+
+
+def test_4096_slots_rebuild_cost_and_results():
+    """The engine at its default slot count: 4096 small spheres admitted, stepped
+    together, one cancelled (a membership change rebuilds the descriptor and
+    tile tables on the host); the host time of a step with a rebuild is logged
+    beside a plain step, and two of the spheres equal their solo runs."""
+    import time
+
+    nt, n = 32, 4096
+    st = seeded_state(nt, L=8, seed=9, dens=1)
+    st2 = seeded_state(nt, L=8, seed=10, dens=1)
+    with K.Engine(max_slots=n) as eng:
+        ids = [eng.admit(nt, 0.5 / nt, *(st2 if k == 7 else st), steps=6) for k in range(n)]
+        assert all(i is not None for i in ids)
+        t0 = time.perf_counter()
+        eng.step(1)  # first step: builds the batch of 4096
+        eng.sync()
+        t_build = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        eng.step(1)
+        eng.sync()
+        t_plain = time.perf_counter() - t0
+        eng.cancel(ids[100])
+        t0 = time.perf_counter()
+        eng.step(1)  # rebuild for 4095
+        eng.sync()
+        t_rebuild = time.perf_counter() - t0
+        eng.step(3)
+        eng.sync()
+        got0, got7 = eng.fetch(ids[0], nt), eng.fetch(ids[7], nt)
+    log(f"[engine 4096] step with first build {t_build * 1e3:.2f} ms, plain step {t_plain * 1e3:.2f} ms, "
+        f"step with rebuild after a cancel {t_rebuild * 1e3:.2f} ms")
+    for got, state in ((got0, st), (got7, st2)):
+        for a, b in zip(got, alone(nt, 0.5 / nt, state, 6)):
+            np.testing.assert_array_equal(a, b)
