@@ -1047,17 +1047,32 @@ __device__ __forceinline__ double rcp64(double x) {
 // 1/x to FP32 precision (the hardware estimate is within 1 ulp).
 __device__ __forceinline__ float rcpT(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcpT(double x) { return rcp64(x); }
+__device__ __forceinline__ float absT(float x) { return fabsf(x); }
+__device__ __forceinline__ double absT(double x) { return fabs(x); }
 
 template <int L, typename T>
 __device__ __forceinline__ T shfl_grp(T v, int srcLaneInGroup, int laneBase) {
     return __shfl(v, laneBase + srcLaneInGroup, 64);
 }
 
-// With the FP32 elimination, the tiles holding the low modes (below
-// kTriFp64Modes) add one pass of iterative refinement with an FP64 residual:
-// their systems are the least diagonally dominant (FP32 Thomas loses up to
-// 1e-4 relative at mode 1 for n_theta = 1024, ~6e-8, FP32 rounding, from
-// mode 16 on; one refinement pass brings mode 1 to ~2e-8).
+// The FP32 elimination forms every pivot without subtraction.  Each row has
+// a, c >= 0 and the diagonal excess m = -b - a - c = n^2 / sin^2 > 0 (pole
+// folds included), so the low modes' systems are nearly singular
+// (m / (a + c) ~ 5e-6 at mode 1, n_theta = 1024): the textbook pivot
+// b_k - a_k ga_{k-1} cancels, and FP32 Thomas loses up to 1e-4 relative
+// there.  Instead, with D_k = -(pivot) > 0,
+//     D_k = c_k + sig_k,  sig_k = m_k + a_k tau_{k-1},  tau = sig / D = 1 - |ga|,
+// and the excess of each normalised row, ep_k = 1 - |al_k| - |ga_k|
+// = (m_k + a_k ep_{k-1}) / D_k, are sums of non-negative terms.  The lane's
+// reduced row keeps its excess E = B - |A| - |C| the same way (from ep at
+// its last row and, for the next lane, R = 1 - Q - S = sum of ep_k times the
+// product of |ga| before k), and parallel cyclic reduction updates it as
+// E' = E + |k1| E_minus + |k2| E_plus, with B' = |A'| + |C'| + E'.  Every
+// pivot then carries FP32 relative accuracy (the usual accurate LDU of a
+// diagonally dominant M-matrix); CPU emulation of this exact sequence gives
+// <= 8.4e-7 relative error at mode 1 up to n_theta = 2048 (6e-8, FP32
+// rounding, at the high modes).  The FP64 elimination keeps the textbook
+// pivots (exact to ~1e-13 there).
 
 // One tile of the tridiagonal solve: G = 256 / L modes of one simulation,
 // L lanes per mode; T is the arithmetic of the elimination (the coefficients
@@ -1066,7 +1081,7 @@ __device__ __forceinline__ T shfl_grp(T v, int srcLaneInGroup, int laneBase) {
 // global reads and writes stay contiguous, and lane l finds its row l K + k
 // at [k][l] (a padded row per k keeps the staging writes spread over the
 // banks).  RESID additionally writes, per mode, ||T x - d|| / ||d|| of the
-// FP64 solution x before it is rounded to FP32 (to SimDesc::resid).  A
+// solution x in T, formed in FP64 (to SimDesc::resid).  A
 // simulation with the FP64 pressure (SIM_P64) keeps x in FP64: each lane
 // stores its rows straight to the double2 spectrum over the finished state
 // copy (parity ^ 1), K consecutive entries per lane, and skips the FP32
@@ -1168,12 +1183,13 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                 c = 0.0;
             }
         };
-        // Solves the group's system for the right-hand side rhs(k) (a float2 per
-        // row of this lane) into (xr, xi), in T: Thomas within the lane, then
-        // parallel cyclic reduction of the lanes' last unknowns.  ACC adds the
-        // solution to (xr, xi) instead (a refinement's correction).
-        auto solve = [&](auto accTag, auto rhs, T* xr, T* xi) {
-            constexpr bool ACC = decltype(accTag)::value;
+        // Solves the group's system for the staged right-hand side (mode 0
+        // dropped: its slot's real part) into (xr, xi), in T: Thomas within
+        // the lane, then parallel cyclic reduction of the lanes' last
+        // unknowns.  SF: the subtraction-free pivots of the FP32 elimination.
+        constexpr bool SF = std::is_same<T, float>::value;
+        T xr[KMAX], xi[KMAX];
+        {
             T al[KMAX], ga[KMAX], dr[KMAX], di[KMAX];
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) {
@@ -1182,27 +1198,52 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                 dr[k] = T(0);
                 di[k] = T(0);
             }
+            // SF running terms: tau_{k-1}, ep_{k-1}, the lane's R = sum over
+            // k < cnt - 1 of ep_k |ga_0 ... ga_{k-1}| with its running product,
+            // and ep at the lane's last row
+            T tau = T(1), epPrev = T(0), R = T(0), prodGa = T(1), eE = T(0);
             // forward elimination within the lane's rows
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) {
                 if (k < cnt) {
-                    double a64, b64, c64;
-                    coef(k, a64, b64, c64);
-                    const T a = (T)a64, b = (T)b64, c = (T)c64;
-                    const float2 f = rhs(k);
-                    const T fr = (T)f.x, fi = (T)f.y;
-                    if (k == 0) {
-                        const T inv = rcpT(b);
-                        al[k] = a * inv;
-                        ga[k] = c * inv;
-                        dr[k] = fr * inv;
-                        di[k] = fi * inv;
+                    const float2 f = zs(g, k, l);
+                    const T fr = (n == 0) ? T(0) : (T)f.x, fi = (T)f.y;
+                    if constexpr (SF) {
+                        double a64, b64, c64;
+                        coef(k, a64, b64, c64);
+                        const T a = (T)a64, c = (T)c64, m = (T)(nn * ss(k, l));
+                        const T sig = m + a * tau;
+                        const T inv = rcpT(c + sig);
+                        const T ep = (m + a * epPrev) * inv;
+                        tau = sig * inv;
+                        epPrev = ep;
+                        al[k] = a * (k == 0 ? T(-1) : al[k > 0 ? k - 1 : 0]) * inv;
+                        ga[k] = -c * inv;
+                        dr[k] = (a * (k == 0 ? T(0) : dr[k > 0 ? k - 1 : 0]) - fr) * inv;
+                        di[k] = (a * (k == 0 ? T(0) : di[k > 0 ? k - 1 : 0]) - fi) * inv;
+                        if (k < cnt - 1) {
+                            R += ep * prodGa;
+                            prodGa *= c * inv;
+                        } else {
+                            eE = ep;
+                        }
                     } else {
-                        const T inv = rcpT(b - a * ga[k - 1]);
-                        al[k] = -a * al[k - 1] * inv;
-                        ga[k] = c * inv;
-                        dr[k] = (fr - a * dr[k - 1]) * inv;
-                        di[k] = (fi - a * di[k - 1]) * inv;
+                        double a64, b64, c64;
+                        coef(k, a64, b64, c64);
+                        const T a = (T)a64, b = (T)b64, c = (T)c64;
+                        if (k == 0) {
+                            const T inv = rcpT(b);
+                            al[k] = a * inv;
+                            ga[k] = c * inv;
+                            dr[k] = fr * inv;
+                            di[k] = fi * inv;
+                        } else {
+                            const T inv = rcpT(b - a * ga[k - 1]);
+                            al[k] = -a * al[k - 1] * inv;
+                            ga[k] = c * inv;
+                            dr[k] = (fr - a * dr[k - 1]) * inv;
+                            di[k] = (fi - a * di[k - 1]) * inv;
+                        }
                     }
                 }
             }
@@ -1213,15 +1254,17 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                 if (k < cnt - 1) {
                     Pr = dr[k] - ga[k] * Pr;
                     Pi = di[k] - ga[k] * Pi;
-                    Q = -al[k] - ga[k] * Q;
+                    if constexpr (!SF) Q = -al[k] - ga[k] * Q;
                     S = -ga[k] * S;
                 }
             }
-            // reduced equation of this lane: A X_{l-1} + B X_l + C X_{l+1} = D
+            // reduced equation of this lane: A X_{l-1} + B X_l + C X_{l+1} = D,
+            // with SF's excess E = B - |A| - |C|
             const int src = (l + 1 < L) ? l + 1 : l;
             const T nPr = shfl_grp<L>(Pr, src, base), nPi = shfl_grp<L>(Pi, src, base);
-            const T nQ = shfl_grp<L>(Q, src, base), nS = shfl_grp<L>(S, src, base);
-            T A = T(0), B = T(1), C = T(0), Dr = T(0), Di = T(0);
+            const T nQ = SF ? T(0) : shfl_grp<L>(Q, src, base), nS = shfl_grp<L>(S, src, base);
+            const T nR = SF ? shfl_grp<L>(R, src, base) : T(0);
+            T A = T(0), B = T(1), C = T(0), Dr = T(0), Di = T(0), E = T(1);
             if (cnt > 0) {
                 T gE = T(0), aE = T(0), drE = T(0), diE = T(0);
 #pragma unroll
@@ -1234,10 +1277,15 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                     }
                 const bool hasNext = (l + 1 < L);
                 A = aE;
-                B = T(1) + (hasNext ? gE * nQ : T(0));
                 C = hasNext ? gE * nS : T(0);
                 Dr = drE - (hasNext ? gE * nPr : T(0));
                 Di = diE - (hasNext ? gE * nPi : T(0));
+                if constexpr (SF) {
+                    E = eE + absT(gE) * (hasNext ? nR : T(1));
+                    B = absT(A) + absT(C) + E;
+                } else {
+                    B = T(1) + (hasNext ? gE * nQ : T(0));
+                }
             }
             // parallel cyclic reduction across the L lanes of the group
 #pragma unroll
@@ -1245,28 +1293,35 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                 const int lm = l - st, lp = l + st;
                 const int sm = lm >= 0 ? lm : l, sp = lp < L ? lp : l;
                 T Am = shfl_grp<L>(A, sm, base), Bm = shfl_grp<L>(B, sm, base);
-                T Cm = shfl_grp<L>(C, sm, base), Dmr = shfl_grp<L>(Dr, sm, base);
-                T Dmi = shfl_grp<L>(Di, sm, base);
-                T Ap = shfl_grp<L>(A, sp, base), Bp = shfl_grp<L>(B, sp, base);
-                T Cp = shfl_grp<L>(C, sp, base), Dpr = shfl_grp<L>(Dr, sp, base);
-                T Dpi = shfl_grp<L>(Di, sp, base);
+                T Dmr = shfl_grp<L>(Dr, sm, base), Dmi = shfl_grp<L>(Di, sm, base);
+                T Bp = shfl_grp<L>(B, sp, base), Cp = shfl_grp<L>(C, sp, base);
+                T Dpr = shfl_grp<L>(Dr, sp, base), Dpi = shfl_grp<L>(Di, sp, base);
+                // SF: the neighbours' excesses; else their other couplings
+                T Xm = SF ? shfl_grp<L>(E, sm, base) : shfl_grp<L>(C, sm, base);
+                T Xp = SF ? shfl_grp<L>(E, sp, base) : shfl_grp<L>(A, sp, base);
                 if (lm < 0) {
                     Am = T(0);
                     Bm = T(1);
-                    Cm = T(0);
                     Dmr = T(0);
                     Dmi = T(0);
+                    Xm = SF ? T(1) : T(0);
                 }
                 if (lp >= L) {
-                    Ap = T(0);
                     Bp = T(1);
                     Cp = T(0);
                     Dpr = T(0);
                     Dpi = T(0);
+                    Xp = SF ? T(1) : T(0);
                 }
                 const T k1 = A * rcpT(Bm), k2 = C * rcpT(Bp);
                 const T nA = -Am * k1, nC = -Cp * k2;
-                const T nB = B - Cm * k1 - Ap * k2;
+                T nB;
+                if constexpr (SF) {
+                    E = E + absT(k1) * Xm + absT(k2) * Xp;
+                    nB = absT(nA) + absT(nC) + E;
+                } else {
+                    nB = B - Xm * k1 - Xp * k2;
+                }
                 const T nDr = Dr - Dmr * k1 - Dpr * k2;
                 const T nDi = Di - Dmi * k1 - Dpi * k2;
                 A = nA;
@@ -1287,24 +1342,22 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
             T vr = Xr, vi = Xi;
 #pragma unroll
             for (int k = KMAX - 1; k >= 0; --k) {
-                if (!ACC) {
-                    xr[k] = T(0);
-                    xi[k] = T(0);
-                }
+                xr[k] = T(0);
+                xi[k] = T(0);
                 if (k < cnt) {
                     if (k < cnt - 1) {
                         vr = dr[k] - al[k] * xmr - ga[k] * vr;
                         vi = di[k] - al[k] * xmi - ga[k] * vi;
                     }
-                    xr[k] = ACC ? xr[k] + vr : vr;
-                    xi[k] = ACC ? xi[k] + vi : vi;
+                    xr[k] = vr;
+                    xi[k] = vi;
                 }
             }
-        };
-        // r = f - T x in FP64, f the staged right-hand side (FP32 values, exact),
-        // one row at a time through `use(k, er, ei, fre, fim)`; the
-        // neighbouring unknowns across lane boundaries come by shuffles
-        auto residual = [&](const T* xr, const T* xi, auto use) {
+        }
+        if constexpr (RESID) {
+            // ||f - T x|| / ||f|| in FP64 over the mode, f the staged
+            // right-hand side (FP32 values, exact); the neighbouring unknowns
+            // across lane boundaries come by shuffles
             const int sprev = (l > 0) ? l - 1 : l, snext = (l + 1 < L) ? l + 1 : l;
             T lastR = T(0), lastI = T(0);
 #pragma unroll
@@ -1320,6 +1373,7 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                 pmr = 0.0;
                 pmi = 0.0;
             }
+            double rsum = 0.0, dsum = 0.0;
 #pragma unroll
             for (int k = 0; k < KMAX; ++k) {
                 if (k < cnt) {
@@ -1333,48 +1387,10 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                     const double fre = (n == 0) ? 0.0 : (double)d.x, fim = (double)d.y;
                     const double er = fre - (a * pr + b * (double)xr[k] + c * qr);
                     const double ei = fim - (a * pi + b * (double)xi[k] + c * qi);
-                    use(k, er, ei, fre, fim);
+                    rsum += er * er + ei * ei;
+                    dsum += fre * fre + fim * fim;
                 }
             }
-        };
-        T xr[KMAX], xi[KMAX];
-        // the staged right-hand side; mode 0 is dropped (its slot's real part)
-        solve(std::false_type{}, [&](int k) {
-            const float2 d = zs(g, k, l);
-            return make_float2((n == 0) ? 0.f : d.x, d.y);
-        }, xr, xi);
-        if constexpr (std::is_same<T, float>::value) {
-            // FP32 elimination: the low modes, the least diagonally dominant,
-            // get one pass of iterative refinement with the residual formed in
-            // FP64 (x += T^-1 r, the correction solved in FP32); uniform over
-            // the block.  The residual is kept in registers when the
-            // residual report still needs the staged right-hand side, else
-            // written over it.
-            if (tile.row0 < kTriFp64Modes) {
-                if constexpr (RESID) {
-                    float2 rk[KMAX];
-                    residual(xr, xi, [&](int k, double er, double ei, double, double) {
-                        rk[k] = make_float2((float)er, (float)ei);
-                    });
-                    solve(std::true_type{}, [&](int k) { return rk[k]; }, xr, xi);
-                } else {
-                    float2 rk[KMAX];
-                    residual(xr, xi, [&](int k, double er, double ei, double, double) {
-                        rk[k] = make_float2((float)er, (float)ei);
-                    });
-#pragma unroll
-                    for (int k = 0; k < KMAX; ++k)
-                        if (k < cnt) zs(g, k, l) = rk[k];
-                    solve(std::true_type{}, [&](int k) { return zs(g, k, l); }, xr, xi);
-                }
-            }
-        }
-        if constexpr (RESID) {
-            double rsum = 0.0, dsum = 0.0;
-            residual(xr, xi, [&](int, double er, double ei, double fre, double fim) {
-                rsum += er * er + ei * ei;
-                dsum += fre * fre + fim * fim;
-            });
             for (int m = 1; m < L; m <<= 1) {
                 rsum += __shfl_xor(rsum, m, 64);
                 dsum += __shfl_xor(dsum, m, 64);

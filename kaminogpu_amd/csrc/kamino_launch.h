@@ -21,13 +21,11 @@ hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, in
                           size_t ldsBytes, int gridCap, hipStream_t st);
 // `parity` names the current state copy: a simulation with the FP64 pressure
 // keeps its solved spectrum over the other one.
-// fp32: run the elimination in FP32 (the FP32 pressure; tiles holding modes
-// below kTriFp64Modes then refine their solution once with an FP64
-// residual); else in FP64 (the FP64 pressure).
+// fp32: run the elimination in FP32 with subtraction-free pivots (the FP32
+// pressure); else in FP64 (the FP64 pressure).
 hipError_t launch_tridiag(int lanesPerMode, int kmax, const SimDesc* sims, const Tile* tiles,
                           int ntiles, int parity, int gridCap, hipStream_t st, bool resid = false,
                           bool fp32 = false);
-constexpr int kTriFp64Modes = 16;
 // p64: the tiles belong to simulations with the FP64 pressure (SIM_P64).
 // pout (one simulation only): write its pressure rows there instead of
 // updating the velocities.
