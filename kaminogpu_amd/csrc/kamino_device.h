@@ -33,7 +33,11 @@ enum RowTab {
     RT_DIVS = 3,      // sin(theta_j + h/2) / (h sin theta_j)
     RT_DIVN = 4,      // sin(theta_j - h/2) / (h sin theta_j)
     RT_INVHS = 5,     // 1 / (h sin theta_j)
-    RT_COUNT = 6
+    // the tridiagonal rows for the FP32 elimination, FP64 values rounded once:
+    RT_TRIA = 6,      // sub-diagonal 1/h^2 - cot(theta_j)/(2h); 0 at j = 0 (pole fold)
+    RT_TRIC = 7,      // super-diagonal 1/h^2 + cot(theta_j)/(2h); 0 at j = nTheta-1
+    RT_TRIS = 8,      // 1 / sin^2(theta_j) (diagonal excess n^2 / sin^2 per mode n)
+    RT_COUNT = 9
 };
 
 // FP64 tables for the tridiagonal rows, each nTheta long.

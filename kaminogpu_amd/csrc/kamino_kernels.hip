@@ -824,12 +824,15 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Lanes per row: a quarter of M rounded up to a power of two, 8..64, so short
-// rows share a wavefront (64 / fft_group lanes groups, one row each).
-__host__ __device__ inline int fft_group(int M) {
-    int g = 8;
-    while (g < 64 && 4 * g < M) g *= 2;
-    return g;
+// Orders the LDS traffic of one lane group of the row FFT (SimDesc::fftShift:
+// a power of two from 8 up to the whole 256-thread block): a wavefront fence
+// for groups within one wavefront, a block barrier for a block-wide group
+// (long rows).  gs is uniform over the block.
+__device__ __forceinline__ void group_sync(int gs) {
+    if (gs > 64)
+        __syncthreads();
+    else
+        wave_sync();
 }
 
 template <int RADIX, int DIR, typename C>
@@ -869,8 +872,9 @@ __device__ __forceinline__ void wave_pass(const C* __restrict__ src, C* __restri
 // Transforms the M complex values at `row` (LDS) in place; `scratch` is the
 // calling lane group's own M-entry LDS row, `tw` the nPhi twiddles
 // exp(-2 pi i n / nPhi) staged in LDS, `lane` the lane within the group of gs
-// lanes.  Every lane of the wavefront calls it (rowless groups pass
-// active = false) so that the fences stay wavefront-uniform.
+// lanes.  Every lane of the wavefront (of the block, for a block-wide group)
+// calls it (rowless groups pass active = false) so that the fences and
+// barriers stay uniform.
 template <int DIR, typename C>
 __device__ void wave_fft(C* row, C* scratch, const C* tw, const SimDesc& s, int lane, int gs,
                          bool active) {
@@ -891,7 +895,7 @@ __device__ void wave_fft(C* row, C* scratch, const C* tw, const SimDesc& s, int 
                 default: wave_pass<5, DIR>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
             }
         }
-        wave_sync();
+        group_sync(gs);
         C* tmp = src;
         src = dst;
         dst = tmp;
@@ -900,7 +904,7 @@ __device__ void wave_fft(C* row, C* scratch, const C* tw, const SimDesc& s, int 
     if (src != row) {
         if (active)
             for (int i = lane; i < M; i += gs) row[i] = src[i];
-        wave_sync();
+        group_sync(gs);
     }
 }
 
@@ -992,7 +996,7 @@ __global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict
                 }
             }
             if (it == 0) __syncthreads();  // twiddles visible to every lane group
-            wave_sync();
+            group_sync(gs);
             wave_fft<-1>(row, scratch, twl, s, lane, gs, active);
         }
         __syncthreads();
@@ -1090,12 +1094,23 @@ constexpr int kTriBatch = 4;  // float4 spectrum loads per thread in flight whil
 
 template <int L, int KMAX, bool RESID, typename T>
 __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const Tile& tile,
-                                         int parity, float2* zsp, double* csp, double* ssp) {
+                                         int parity, float2* zsp, double* csp, double* ssp,
+                                         float* fsp) {
     constexpr int G = 256 / L;  // modes per block
     (void)G;
+    // SF: the subtraction-free pivots of the FP32 elimination, from FP32 row
+    // tables; the FP64 row terms are staged for the FP64 elimination and for
+    // the residual report
+    constexpr bool SF = std::is_same<T, float>::value;
+    constexpr bool kStage64 = !SF || RESID;
+    constexpr int NR = KMAX * (L + 1);
     auto zs = [&](int gg, int k, int ll) -> float2& { return zsp[(gg * KMAX + k) * (L + 1) + ll]; };
     auto cs = [&](int k, int ll) -> double& { return csp[k * (L + 1) + ll]; };
     auto ss = [&](int k, int ll) -> double& { return ssp[k * (L + 1) + ll]; };
+    // FP32 rows: sub-diagonal, super-diagonal (pole folds included), 1/sin^2
+    auto fa = [&](int k, int ll) -> float& { return fsp[k * (L + 1) + ll]; };
+    auto fc = [&](int k, int ll) -> float& { return fsp[NR + k * (L + 1) + ll]; };
+    auto fs = [&](int k, int ll) -> float& { return fsp[2 * NR + k * (L + 1) + ll]; };
     const int tid = threadIdx.x;
     const int l = tid % L;               // lane within the mode group
     const int g = tid / L;               // mode within the block
@@ -1124,22 +1139,46 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
         // staging: a thread's loads of a batch all go out before its first
         // LDS write; reads past the tables or the spectrum return 0 and are
         // not kept
-        const Rsrc rowR = make_rsrc(reinterpret_cast<const float*>(s.rowd), 2 * RD_COUNT * nT);
-        for (int i0 = tid; i0 < nT; i0 += 4 * 256) {
-            double c[4], v[4];
+        if constexpr (kStage64) {
+            const Rsrc rowR = make_rsrc(reinterpret_cast<const float*>(s.rowd), 2 * RD_COUNT * nT);
+            for (int i0 = tid; i0 < nT; i0 += 4 * 256) {
+                double c[4], v[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                c[q] = ldd(rowR, RD_COT2H * nT + i0 + q * 256);
-                v[q] = ldd(rowR, RD_INVSIN2 * nT + i0 + q * 256);
+                for (int q = 0; q < 4; ++q) {
+                    c[q] = ldd(rowR, RD_COT2H * nT + i0 + q * 256);
+                    v[q] = ldd(rowR, RD_INVSIN2 * nT + i0 + q * 256);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + q * 256;
+                    if (i >= nT) break;
+                    int li, ki;
+                    split(i, li, ki);
+                    cs(ki, li) = c[q];
+                    ss(ki, li) = v[q];
+                }
             }
+        }
+        if constexpr (SF) {
+            const Rsrc rowF = make_rsrc(s.rowf, RT_COUNT * nT);
+            for (int i0 = tid; i0 < nT; i0 += 4 * 256) {
+                float a[4], c[4], v[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const int i = i0 + q * 256;
-                if (i >= nT) break;
-                int li, ki;
-                split(i, li, ki);
-                cs(ki, li) = c[q];
-                ss(ki, li) = v[q];
+                for (int q = 0; q < 4; ++q) {
+                    a[q] = ldf(rowF, RT_TRIA * nT + i0 + q * 256);
+                    c[q] = ldf(rowF, RT_TRIC * nT + i0 + q * 256);
+                    v[q] = ldf(rowF, RT_TRIS * nT + i0 + q * 256);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + q * 256;
+                    if (i >= nT) break;
+                    int li, ki;
+                    split(i, li, ki);
+                    fa(ki, li) = a[q];
+                    fc(ki, li) = c[q];
+                    fs(ki, li) = v[q];
+                }
             }
         }
         // the tile's modes are consecutive spectrum rows: pairs of entries
@@ -1166,6 +1205,7 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
         __syncthreads();
         const double h2 = s.invH2;
         const double nn = (n == 0) ? (double)s.M * (double)s.M : (double)n * (double)n;
+        const float nnf = (float)nn;  // exact: n^2 <= 2^22
         // row i = r0 + k of this lane, a x_{i-1} + b x_i + c x_{i+1} = f, in FP64
         // (the pole rows folded, KaminoSolver.cu:117-163)
         auto coef = [&](int k, double& a, double& b, double& c) {
@@ -1186,8 +1226,7 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
         // Solves the group's system for the staged right-hand side (mode 0
         // dropped: its slot's real part) into (xr, xi), in T: Thomas within
         // the lane, then parallel cyclic reduction of the lanes' last
-        // unknowns.  SF: the subtraction-free pivots of the FP32 elimination.
-        constexpr bool SF = std::is_same<T, float>::value;
+        // unknowns.
         T xr[KMAX], xi[KMAX];
         {
             T al[KMAX], ga[KMAX], dr[KMAX], di[KMAX];
@@ -1209,9 +1248,7 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                     const float2 f = zs(g, k, l);
                     const T fr = (n == 0) ? T(0) : (T)f.x, fi = (T)f.y;
                     if constexpr (SF) {
-                        double a64, b64, c64;
-                        coef(k, a64, b64, c64);
-                        const T a = (T)a64, c = (T)c64, m = (T)(nn * ss(k, l));
+                        const T a = fa(k, l), c = fc(k, l), m = nnf * fs(k, l);
                         const T sig = m + a * tau;
                         const T inv = rcpT(c + sig);
                         const T ep = (m + a * epPrev) * inv;
@@ -1433,9 +1470,12 @@ template <int L, int KMAX, bool RESID, typename T>
 __global__ __launch_bounds__(256) void k_tridiag(const SimDesc* __restrict__ sims,
                                                  const Tile* __restrict__ tiles, int ntiles, int parity) {
     __shared__ float2 zs[(256 / L) * KMAX * (L + 1)];
-    __shared__ double cs[KMAX * (L + 1)], ss[KMAX * (L + 1)];
+    constexpr bool SF = std::is_same<T, float>::value;
+    constexpr int NR = KMAX * (L + 1);
+    __shared__ double cs[(!SF || RESID) ? NR : 1], ss[(!SF || RESID) ? NR : 1];
+    __shared__ float fr[SF ? 3 * NR : 1];
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles)
-        tri_tile<L, KMAX, RESID, T>(sims, tiles[t], parity, zs, cs, ss);
+        tri_tile<L, KMAX, RESID, T>(sims, tiles[t], parity, zs, cs, ss, fr);
 }
 
 // Every class with K = 8 rows per lane in one launch: a tile carries its lanes
@@ -1447,14 +1487,16 @@ __global__ __launch_bounds__(256) void k_tridiag_mixed(const SimDesc* __restrict
                                                        int parity) {
     constexpr int K = 8, ZS = 32 * K * 9, CS = K * 65;  // largest over L = 8 ... 64
     __shared__ float2 zs[ZS];
-    __shared__ double cs[CS], ss[CS];
+    constexpr bool SF = std::is_same<T, float>::value;
+    __shared__ double cs[(!SF || RESID) ? CS : 1], ss[(!SF || RESID) ? CS : 1];
+    __shared__ float fr[SF ? 3 * CS : 1];
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
         switch (tile.col0) {
-            case 8: tri_tile<8, K, RESID, T>(sims, tile, parity, zs, cs, ss); break;
-            case 16: tri_tile<16, K, RESID, T>(sims, tile, parity, zs, cs, ss); break;
-            case 32: tri_tile<32, K, RESID, T>(sims, tile, parity, zs, cs, ss); break;
-            default: tri_tile<64, K, RESID, T>(sims, tile, parity, zs, cs, ss); break;
+            case 8: tri_tile<8, K, RESID, T>(sims, tile, parity, zs, cs, ss, fr); break;
+            case 16: tri_tile<16, K, RESID, T>(sims, tile, parity, zs, cs, ss, fr); break;
+            case 32: tri_tile<32, K, RESID, T>(sims, tile, parity, zs, cs, ss, fr); break;
+            default: tri_tile<64, K, RESID, T>(sims, tile, parity, zs, cs, ss, fr); break;
         }
     }
 }

@@ -65,7 +65,11 @@ static size_t adv_lds(int R, int W, int SW, bool geometric) {
 }
 // Lanes per row in the row FFT (a quarter of M, 8..64, power of two); a
 // 256-thread block holds 256 / group lane groups, each with a scratch row.
+// Long rows (M >= 1024) take the whole block per row, its passes ordered by
+// block barriers: a 1024 x 2048 sphere has only four rows per CU, so with
+// one wavefront per row each SIMD would hold a single transform's chain.
 static int fft_group_host(int M) {
+    if (M >= 1024) return 256;
     int g = 8;
     while (g < 64 && 4 * g < M) g *= 2;
     return g;
@@ -131,10 +135,11 @@ int make_plan(int nT, int nP, Plan& p) {
     // FFT tile: at most 8 rows (+1 halo row for the inverse, + scratch rows) in ~48 KiB
     int Rf = (int)(49152 / (8 * (size_t)p.M)) - 3 - 256 / fft_group_host(p.M);
     Rf = Rf < 1 ? 1 : (Rf > 8 ? 8 : Rf);
-    // long rows (M = 1024: 16 KiB of twiddles alone) still get two rows per
-    // tile, in 72 KiB: with one, three of the four lane groups sit idle
-    // (measured at 1024x2048: both FFT kernels 0.069 -> 0.046 ms)
-    if (Rf < 2 && fft_lds(2, p.M) <= 160 * 1024) Rf = 2;
+    // long rows (a block per row): one row per tile.  The inverse transform
+    // then repeats its halo row, but twice as many tiles run at once
+    // (measured, ms per step at one sphere: 1024 x 2048 0.0811 with two rows
+    // -> 0.0798, 2048 x 4096 0.314 -> 0.272)
+    if (fft_group_host(p.M) == 256) Rf = 1;
     p.fftRows = Rf > nT ? nT : Rf;
     // tridiagonal: L lanes per mode, at most K rows per lane
     if (nT <= 64) { p.triL = 8; p.triK = 8; }
@@ -180,8 +185,10 @@ int adjust_plan(Plan& p, int advRows, int fftRows) {
 int check_precision(int precision, const Plan& plan) {
     if (precision != KAM_PREC_FP32 && precision != KAM_PREC_FP64_PRESSURE)
         return set_err(KAM_EINVAL, "unknown precision %d", precision);
-    if (precision == KAM_PREC_FP64_PRESSURE && fft_lds(1, plan.M, true) > kLdsMax)
-        return set_err(KAM_EINVAL, "the FP64 pressure needs n_theta <= 1024 (its FFT rows must fit in LDS)");
+    // the FP64 inverse transform's rows must fit in LDS, and the FP64
+    // elimination is built for at most 16 rows per lane
+    if (precision == KAM_PREC_FP64_PRESSURE && (plan.nT > 1024 || fft_lds(1, plan.M, true) > kLdsMax))
+        return set_err(KAM_EINVAL, "the FP64 pressure needs n_theta <= 1024");
     return KAM_OK;
 }
 
@@ -212,6 +219,11 @@ void compute_tables(int nT, double dt, double radius, float* rowf, double* rowd,
         rowd[RD_COT2H * nT + j] = cc / (2.0 * h * sc);
         rowd[RD_INVSIN2 * nT + j] = 1.0 / (sc * sc);
         rowd[RD_INVHS * nT + j] = 1.0 / (h * sc);
+        // the same FP64 terms as the FP64 elimination forms (1/h^2 is SimDesc::invH2)
+        const double h2 = 1.0 / (h * h), ct = rowd[RD_COT2H * nT + j];
+        rowf[RT_TRIA * nT + j] = (j == 0) ? 0.f : (float)(h2 - ct);
+        rowf[RT_TRIC * nT + j] = (j == nT - 1) ? 0.f : (float)(h2 + ct);
+        rowf[RT_TRIS * nT + j] = (float)rowd[RD_INVSIN2 * nT + j];
     }
     for (int n = 0; n < nP; ++n) {
         const double a = 2.0 * kPi * n / nP;
