@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven-steps", type=int, default=200, help="config-3 batch steps (0 skips)")
     ap.add_argument("--large-steps", type=int, default=500, help="config-2 steps at 1024x2048 (0 skips)")
+    ap.add_argument("--large-transform", choices=["auto", "fft", "dft"], default="auto",
+                    help="phi transform of the config-2 leg: the autotuner's pick, or forced")
     ap.add_argument("--serve-requests", type=int, default=10000, help="config-4 requests (0 skips)")
     ap.add_argument("--launch", choices=["graph", "direct"], default="direct",
                     help="engine steps as direct kernel launches (default: each hipGraph launch "
@@ -191,12 +193,16 @@ def run_served(K, n):
     return out
 
 
-def run_large(K, steps, cache, nt=1024):
+def run_large(K, steps, cache, nt=1024, transform="auto"):
     """Config 2: one 1024x2048 sphere, dt = 0.5/nTheta, seeded flow with L = 32,
-    stepped alone (plain launches, FP32 fields, FP64 tridiagonal); tiles
-    autotuned on the sphere itself."""
+    stepped alone (plain launches, FP32 fields and pressure); tiles autotuned
+    on the sphere itself.  transform "fft" or "dft" overrides the autotuner's
+    choice of the phi transform (the MFMA DFT products, for profiling them)."""
     dt = 0.5 / nt
     tiles, _ = tune(K, nt, 1, cache)
+    if transform != "auto":
+        K.tune_set_transform(nt, K.TRANSFORM_DFT if transform == "dft" else K.TRANSFORM_FFT)
+        tiles["transform"] = "dft_mfma" if transform == "dft" else "fft"
     log(f"large tiles: {tiles}")
     with K.Sim(nt, dt) as s:
         s.set_fields(*init_state(nt, 32, 2, 1))
@@ -336,7 +342,7 @@ def main():
     log(f"batched x{B} FP64 pressure: {p64['cell_steps_per_s']:.3e} cell-steps/s, "
         f"{p64['ms_per_step']:.4f} ms/step, kernel ms {np.round(kern64, 4)}")
 
-    large = run_large(K, a.large_steps, a.tune_cache) if a.large_steps > 0 else None
+    large = run_large(K, a.large_steps, a.tune_cache, transform=a.large_transform) if a.large_steps > 0 else None
     uneven = run_uneven(K, a.uneven_steps, a.tune_cache) if a.uneven_steps > 0 else None
     served = run_served(K, a.serve_requests) if a.serve_requests > 0 else None
 
