@@ -1028,7 +1028,8 @@ __global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict
 // k_tridiag: T_n p-hat = d-hat for every Fourier mode n (crKernel, tdm.cu:3-96,
 // with the rows of precomputeABCKernel, KaminoSolver.cu:117-163: sub-diagonal
 // a = 1/h^2 - cot/(2h), diagonal b = -2/h^2 - n^2/sin^2, super-diagonal
-// c = 1/h^2 + cot/(2h), pole rows folded b += a / b += c).  Arithmetic is FP64.
+// c = 1/h^2 + cot/(2h), pole rows folded b += a / b += c), in FP32 with
+// subtraction-free pivots or in FP64 (the FP64 pressure).
 //
 // L lanes share one mode; lane l owns K consecutive rows and runs Thomas on
 // them in registers, keeping each row in the form
