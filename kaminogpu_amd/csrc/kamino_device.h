@@ -6,7 +6,8 @@
 //   state[b][0] u_phi   [nTheta][nPhi]      b = 0, 1 (ping-pong per step)
 //   state[b][1] u_theta [nTheta][nPhi]      (row nTheta-1 is padding, zeroed)
 //   state[b][2] rho     [nTheta][nPhi]
-//   spec        [M][nTheta] float2          half spectrum, mode-major, M = nPhi/2;
+//   spec        M x nTheta float2           half spectrum, M = nPhi/2, in blocks of
+//                                           four theta rows (spec_index below);
 //                                           slot 0 packs (mode 0, Nyquist mode)
 // 28 bytes per cell in total (plus one 256-byte block for the watchdog word of
 // a stand-alone simulation; engine simulations keep theirs in a slot table).
@@ -61,7 +62,7 @@ struct SimDesc {
     float invH;            // 1 / h
     double invH2;          // 1 / h^2
     float* state[2][3];    // [parity][field]
-    float* spec;           // float2 [M][nTheta]
+    float* spec;           // float2 M x nTheta, row-blocked (spec_index)
     const float* rowf;     // RT_COUNT * nTheta
     const double* rowd;    // RD_COUNT * nTheta
     const float* twN;      // float2 [nPhi]: exp(-2 pi i n / nPhi)
@@ -91,6 +92,19 @@ struct SimDesc {
     const uint16_t* dftInv[2];
     int dftK;
 };
+
+// The FP32 half spectrum is stored in blocks of kSpecRows theta rows: block b
+// holds rows 4b .. 4b + rb - 1 (rb = min(4, nTheta - 4b)), mode-major within
+// the block, so mode n of row j is complex entry
+//     4b M + n rb + (j - 4b),   b = j / 4.
+// A transform tile of four rows reads or writes one contiguous run of
+// 4 M entries, and the solve reads a tile's modes as runs of (modes x 4 rows).
+constexpr int kSpecRows = 4;
+__host__ __device__ inline size_t spec_index(int n, int j, int M, int nT) {
+    const int j0 = j & ~(kSpecRows - 1);
+    const int rb = (nT - j0 < kSpecRows) ? nT - j0 : kSpecRows;
+    return (size_t)j0 * M + (size_t)n * rb + (j - j0);
+}
 
 // One unit of work: rows [row0, row0 + rows) of simulation `sim` (or, for the
 // tridiagonal kernel, modes [row0, row0 + rows)); advection tiles also start

@@ -46,7 +46,7 @@ __device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo) {
 template <bool INV>
 __device__ __forceinline__ float load_a(const SimDesc& s, Rsrc a, int j, int k) {
     if (k >= s.nPhi || j >= s.nTheta) return 0.f;
-    if constexpr (INV) return ldf(a, 2 * ((k >> 1) * s.nTheta + j) + (k & 1));
+    if constexpr (INV) return ldf(a, 2 * (int)spec_index(k >> 1, j, s.M, s.nTheta) + (k & 1));
     else return ldf(a, j * s.nPhi + k);
 }
 
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_dft_gemm(const SimDesc* __restrict__ si
 
         // per-row scale: max |A[j][.]| over the whole row.  Forward: each
         // wavefront walks rows with its lanes along the row (contiguous
-        // reads), then reduces across lanes; inverse (mode-major spectrum):
+        // reads), then reduces across lanes; inverse (the spectrum in row blocks):
         // lanes along the rows, four partial maxima per row.
         if constexpr (INV) {
             const int r = tid & 63, q = tid >> 6;
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void k_dft_gemm(const SimDesc* __restrict__ si
                     if constexpr (INV)
                         pOut[(size_t)j * nP + c] = val;
                     else
-                        specOut[2 * ((size_t)(c >> 1) * nT + j) + (c & 1)] = val * s.invNP;
+                        specOut[2 * spec_index(c >> 1, j, s.M, nT) + (c & 1)] = val * s.invNP;
                 }
         }
         __syncthreads();

@@ -937,9 +937,10 @@ __device__ __forceinline__ void stage_twiddles(float2* twl, const SimDesc& s, in
 // k_div_fft: divergence (fillDivergenceKernel, KaminoCore.cu:587-638) of R
 // rows, packed two reals per complex (z_k = d_2k + i d_2k+1), FFT of length
 // M, split into the half spectrum X_n = E_n + exp(-i pi n / M) O_n and scaled
-// by 1/nPhi.  Mode n of row j goes to spec[n][j] (mode-major); slot 0 holds
-// (d-hat_0, d-hat_Nyquist).  This is the reference's cuFFT CUFFT_INVERSE
-// call plus shiftFKernel (:640-656): d-hat_n = (1/nPhi) sum_i d_i e^{-i n i h}.
+// by 1/nPhi.  Mode n of row j goes to spec[spec_index(n, j)] (row blocks of
+// four, mode-major within a block); slot 0 holds (d-hat_0, d-hat_Nyquist).
+// This is the reference's cuFFT CUFFT_INVERSE call plus shiftFKernel
+// (:640-656): d-hat_n = (1/nPhi) sum_i d_i e^{-i n i h}.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict__ sims,
                                                  const Tile* __restrict__ tiles, int ntiles,
@@ -1018,7 +1019,7 @@ __global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict
                 const float2 X = cadd(E, cmul(tw[n], O));
                 out = cscale(X, scale);
             }
-            spec[(size_t)n * nT + j] = out;
+            spec[spec_index(n, j, M, nT)] = out;
         }
         __syncthreads();
     }
@@ -1182,24 +1183,54 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                 }
             }
         }
-        // the tile's modes are consecutive spectrum rows: pairs of entries
-        // as float4 (row0 * nT is even, row0 being a multiple of 256 / L)
+        // The tile's entries in spectrum order (spec_index): row block by row
+        // block, in each the tile's modes as runs of the block's rb rows
+        // (rb = 4 but in the last block), so entry e of the tile lies at
+        // offset 4 b M + row0 rb + (e - b rows 4) of its block b.  Pairs of
+        // entries go as float4: offsets of even entries are even (row0 is a
+        // multiple of 256 / L, full blocks hold an even count) and a pair
+        // never straddles two blocks.
         const Rsrc specR = make_rsrc(s.spec, 2 * s.M * nT);
-        const int total = tile.rows * nT, b2 = 2 * tile.row0 * nT;
+        const int total = tile.rows * nT, blkN = tile.rows * kSpecRows, Ms = s.M;
+        const unsigned magicB = 0xffffffffu / (unsigned)blkN + 1u;  // e / blkN, e < 2^16, blkN >= 4
+        auto locate = [&](int e, int& g, int& i) -> int {
+            const int b = (int)__umulhi((unsigned)e, magicB);
+            const int rem = e - b * blkN, jb = b * kSpecRows;
+            const int rb = min(kSpecRows, nT - jb);
+            g = (rb == 4) ? rem >> 2 : (rb == 2) ? rem >> 1 : (rb == 1) ? rem : (int)__umulhi((unsigned)rem, 0x55555556u);
+            i = jb + rem - g * rb;
+            return jb * Ms + tile.row0 * rb + rem;
+        };
+        // entry e + 1 of an even e: the next row of the block, or the next
+        // mode's first row in it
+        auto next = [&](int& g, int& i) {
+            const int jb = i & ~(kSpecRows - 1);
+            if (i + 1 < min(jb + kSpecRows, nT)) {
+                ++i;
+            } else {
+                ++g;
+                i = jb;
+            }
+        };
         for (int e0 = 2 * tid; e0 < total; e0 += 2 * 256 * kTriBatch) {
             float4 v[kTriBatch];
+            int gq[kTriBatch], iq[kTriBatch];
 #pragma unroll
-            for (int q = 0; q < kTriBatch; ++q) v[q] = ldf4(specR, b2 + 2 * (e0 + 512 * q));
+            for (int q = 0; q < kTriBatch; ++q) {
+                const int e = e0 + 512 * q;
+                const int a = locate(e, gq[q], iq[q]);
+                v[q] = ldf4(specR, e < total ? 2 * a : -1);  // entries past the tile read as 0, unused
+            }
 #pragma unroll
             for (int q = 0; q < kTriBatch; ++q)
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const int e = e0 + 512 * q + h;
                     if (e < total) {
-                        const int ge = (int)__umulhi((unsigned)(2 * e), s.magicP), i = e - ge * nT;
+                        if (h) next(gq[q], iq[q]);
                         int li, ki;
-                        split(i, li, ki);
-                        zs(ge, ki, li) = h ? make_float2(v[q].z, v[q].w) : make_float2(v[q].x, v[q].y);
+                        split(iq[q], li, ki);
+                        zs(gq[q], ki, li) = h ? make_float2(v[q].z, v[q].w) : make_float2(v[q].x, v[q].y);
                     }
                 }
         }
@@ -1446,21 +1477,24 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
         }
         __syncthreads();
         if (p64) return;  // the solution is already stored (uniform over the block)
-        // write-back, pairs of entries as float4 where both lie in the tile
-        float* __restrict__ blk = s.spec + b2;
-        auto staged = [&](int e) -> float2 {
-            const int ge = (int)__umulhi((unsigned)(2 * e), s.magicP), i = e - ge * nT;
+        // write-back in spectrum order, pairs of entries as float4 where both
+        // lie in the tile
+        float* __restrict__ spw = s.spec;
+        auto staged = [&](int g, int i) -> float2 {
             int li, ki;
             split(i, li, ki);
-            return zs(ge, ki, li);
+            return zs(g, ki, li);
         };
         for (int e = 2 * tid; e < total; e += 512) {
-            const float2 x = staged(e);
+            int g, i;
+            const int a = locate(e, g, i);
+            const float2 x = staged(g, i);
             if (e + 1 < total) {
-                const float2 y = staged(e + 1);
-                *reinterpret_cast<float4*>(blk + 2 * e) = make_float4(x.x, x.y, y.x, y.y);
+                next(g, i);
+                const float2 y = staged(g, i);
+                *reinterpret_cast<float4*>(spw + 2 * (size_t)a) = make_float4(x.x, x.y, y.x, y.y);
             } else {
-                *reinterpret_cast<float2*>(blk + 2 * e) = x;
+                *reinterpret_cast<float2*>(spw + 2 * (size_t)a) = x;
             }
         }
     }
@@ -1571,8 +1605,15 @@ __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restri
                 const int idx = idx0 + q * nthr;
                 const int n = (Rp == 1) ? idx : (int)__umulhi((unsigned)idx, magicR), j = j0 + idx - n * Rp;
                 // past the spectrum (idx beyond the tile, or M - n = M) reads as 0
-                load_spec(specR, n * nT + j, Pn[q]);
-                load_spec(specR, (M - n) * nT + j, Pm[q]);
+                // or is not used; the FP64 p-hat is mode-major, the FP32 one
+                // in row blocks
+                if constexpr (kP64) {
+                    load_spec(specR, n * nT + j, Pn[q]);
+                    load_spec(specR, (M - n) * nT + j, Pm[q]);
+                } else {
+                    load_spec(specR, (int)spec_index(n, j, M, nT), Pn[q]);
+                    load_spec(specR, (int)spec_index(M - n, j, M, nT), Pm[q]);
+                }
             }
 #pragma unroll
             for (int q = 0; q < kSpecBatch; ++q) {

@@ -1005,9 +1005,19 @@ int kam_sim_get_pressure(kam_sim* s, float* p) {
 int kam_sim_get_spectrum(kam_sim* s, float* spec) {
     if (!s || !spec) return set_err(KAM_EINVAL, "null argument");
     KAM_HIP(hipSetDevice(s->sd.device));
-    const size_t n = (size_t)s->plan.M * s->plan.nT * 2;
-    KAM_HIP(hipMemcpyAsync(spec, s->desc.spec, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    const int M = s->plan.M, nT = s->plan.nT;
+    const size_t n = (size_t)M * nT * 2;
+    std::vector<float> blocked(n);
+    KAM_HIP(hipMemcpyAsync(blocked.data(), s->desc.spec, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     KAM_HIP(hipStreamSynchronize(s->stream));
+    // the device keeps the spectrum in row blocks (spec_index); the caller
+    // gets it mode-major
+    for (int m = 0; m < M; ++m)
+        for (int j = 0; j < nT; ++j) {
+            const size_t src = 2 * spec_index(m, j, M, nT), dst = 2 * ((size_t)m * nT + j);
+            spec[dst] = blocked[src];
+            spec[dst + 1] = blocked[src + 1];
+        }
     return KAM_OK;
 }
 
