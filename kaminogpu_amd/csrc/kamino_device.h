@@ -6,10 +6,12 @@
 //   state[b][0] u_phi   [nTheta][nPhi]      b = 0, 1 (ping-pong per step)
 //   state[b][1] u_theta [nTheta][nPhi]      (row nTheta-1 is padding, zeroed)
 //   state[b][2] rho     [nTheta][nPhi]
-//   spec        M x nTheta float2           half spectrum, M = nPhi/2, in blocks of
-//                                           four theta rows (spec_index below);
+//   spec        M x nTheta4 float2          half spectrum, M = nPhi/2, in blocks of
+//                                           four theta rows (spec_index below;
+//                                           nTheta4 = nTheta rounded up to 4);
 //                                           slot 0 packs (mode 0, Nyquist mode)
-// 28 bytes per cell in total (plus one 256-byte block for the watchdog word of
+// 28 bytes per cell in total (a little more when nTheta is not a multiple of
+// 4; plus one 256-byte block for the watchdog word of
 // a stand-alone simulation; engine simulations keep theirs in a slot table).
 // With the FP64 pressure (SIM_P64), the solved spectrum p-hat is written as
 // double2 [M][nTheta] over the state copy the step has finished with (the
@@ -62,7 +64,7 @@ struct SimDesc {
     float invH;            // 1 / h
     double invH2;          // 1 / h^2
     float* state[2][3];    // [parity][field]
-    float* spec;           // float2 M x nTheta, row-blocked (spec_index)
+    float* spec;           // float2 M x nTheta4, row-blocked (spec_index)
     const float* rowf;     // RT_COUNT * nTheta
     const double* rowd;    // RD_COUNT * nTheta
     const float* twN;      // float2 [nPhi]: exp(-2 pi i n / nPhi)
@@ -93,17 +95,18 @@ struct SimDesc {
     int dftK;
 };
 
-// The FP32 half spectrum is stored in blocks of kSpecRows theta rows: block b
-// holds rows 4b .. 4b + rb - 1 (rb = min(4, nTheta - 4b)), mode-major within
-// the block, so mode n of row j is complex entry
-//     4b M + n rb + (j - 4b),   b = j / 4.
-// A transform tile of four rows reads or writes one contiguous run of
-// 4 M entries, and the solve reads a tile's modes as runs of (modes x 4 rows).
+// The FP32 half spectrum is stored in blocks of kSpecRows = 4 theta rows,
+// mode-major within a block, so mode n of row j is complex entry
+//     4b M + 4n + (j - 4b),   b = j / 4;
+// rows past nTheta up to the next multiple of 4 are padding (never written
+// or read as values).  A transform tile of four rows reads or writes one
+// contiguous run of 4 M entries, and the solve reads a tile's modes as runs
+// of (modes x 4 rows).
 constexpr int kSpecRows = 4;
-__host__ __device__ inline size_t spec_index(int n, int j, int M, int nT) {
+__host__ __device__ inline int spec_rows(int nT) { return (nT + kSpecRows - 1) & ~(kSpecRows - 1); }
+__host__ __device__ inline size_t spec_index(int n, int j, int M) {
     const int j0 = j & ~(kSpecRows - 1);
-    const int rb = (nT - j0 < kSpecRows) ? nT - j0 : kSpecRows;
-    return (size_t)j0 * M + (size_t)n * rb + (j - j0);
+    return (size_t)j0 * M + (size_t)n * kSpecRows + (j - j0);
 }
 
 // One unit of work: rows [row0, row0 + rows) of simulation `sim` (or, for the

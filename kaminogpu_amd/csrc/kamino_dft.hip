@@ -46,7 +46,7 @@ __device__ __forceinline__ void split_h(float x, _Float16& hi, _Float16& lo) {
 template <bool INV>
 __device__ __forceinline__ float load_a(const SimDesc& s, Rsrc a, int j, int k) {
     if (k >= s.nPhi || j >= s.nTheta) return 0.f;
-    if constexpr (INV) return ldf(a, 2 * (int)spec_index(k >> 1, j, s.M, s.nTheta) + (k & 1));
+    if constexpr (INV) return ldf(a, 2 * (int)spec_index(k >> 1, j, s.M) + (k & 1));
     else return ldf(a, j * s.nPhi + k);
 }
 
@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_dft_gemm(const SimDesc* __restrict__ si
         const int j0 = tile.row0, c0 = tile.col0;
         // A: the divergence rows over the finished state copy (forward) or
         // the spectrum (inverse); B: this direction's FP16 planes, [N][K]
-        const Rsrc a = INV ? make_rsrc(s.spec, 2 * s.M * nT) : make_rsrc(s.state[parity ^ 1][F_UPHI], nT * nP);
+        const Rsrc a = INV ? make_rsrc(s.spec, 2 * s.M * spec_rows(nT)) : make_rsrc(s.state[parity ^ 1][F_UPHI], nT * nP);
         const _Float16* __restrict__ gHi = reinterpret_cast<const _Float16*>(INV ? s.dftInv[0] : s.dftFwd[0]);
         const _Float16* __restrict__ gLo = reinterpret_cast<const _Float16*>(INV ? s.dftInv[1] : s.dftFwd[1]);
 
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void k_dft_gemm(const SimDesc* __restrict__ si
                     if constexpr (INV)
                         pOut[(size_t)j * nP + c] = val;
                     else
-                        specOut[2 * spec_index(c >> 1, j, s.M, nT) + (c & 1)] = val * s.invNP;
+                        specOut[2 * spec_index(c >> 1, j, s.M) + (c & 1)] = val * s.invNP;
                 }
         }
         __syncthreads();

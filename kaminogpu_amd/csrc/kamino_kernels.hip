@@ -1019,7 +1019,7 @@ __global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict
                 const float2 X = cadd(E, cmul(tw[n], O));
                 out = cscale(X, scale);
             }
-            spec[spec_index(n, j, M, nT)] = out;
+            spec[spec_index(n, j, M)] = out;
         }
         __syncthreads();
     }
@@ -1183,56 +1183,50 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                 }
             }
         }
-        // The tile's entries in spectrum order (spec_index): row block by row
-        // block, in each the tile's modes as runs of the block's rb rows
-        // (rb = 4 but in the last block), so entry e of the tile lies at
-        // offset 4 b M + row0 rb + (e - b rows 4) of its block b.  Pairs of
-        // entries go as float4: offsets of even entries are even (row0 is a
-        // multiple of 256 / L, full blocks hold an even count) and a pair
-        // never straddles two blocks.
-        const Rsrc specR = make_rsrc(s.spec, 2 * s.M * nT);
-        const int total = tile.rows * nT, blkN = tile.rows * kSpecRows, Ms = s.M;
-        const unsigned magicB = 0xffffffffu / (unsigned)blkN + 1u;  // e / blkN, e < 2^16, blkN >= 4
-        auto locate = [&](int e, int& g, int& i) -> int {
-            const int b = (int)__umulhi((unsigned)e, magicB);
-            const int rem = e - b * blkN, jb = b * kSpecRows;
-            const int rb = min(kSpecRows, nT - jb);
-            g = (rb == 4) ? rem >> 2 : (rb == 2) ? rem >> 1 : (rb == 1) ? rem : (int)__umulhi((unsigned)rem, 0x55555556u);
-            i = jb + rem - g * rb;
-            return jb * Ms + tile.row0 * rb + rem;
+        // The tile's entries in spectrum order (spec_index): block by block of
+        // four rows, in each the tile's modes as runs of four rows.  Pair p
+        // of the tile (two entries, one float4) is mode row0 + g, rows i and
+        // i + 1 of block b:  b = p / (2 rows), w = p - 2 rows b, g = w / 2,
+        // i = 4 b + 2 (w % 2), at float2 offset 4 b M + 4 row0 + 2 w (even, so
+        // float4-aligned).  Rows past nTheta are padding and are skipped.
+        const Rsrc specR = make_rsrc(s.spec, 2 * s.M * spec_rows(nT));
+        const int pairN = 2 * tile.rows, nPairs = tile.rows * (spec_rows(nT) >> 1), Ms = s.M;
+        const unsigned magicB = 0xffffffffu / (unsigned)pairN + 1u;  // p / pairN, p < 2^16, pairN >= 2
+        auto pair = [&](int p, int& g, int& i) -> int {
+            const int b = (int)__umulhi((unsigned)p, magicB);
+            const int w = p - b * pairN;
+            g = w >> 1;
+            i = 4 * b + ((w & 1) << 1);
+            return 4 * b * Ms + 4 * tile.row0 + 2 * w;
         };
-        // entry e + 1 of an even e: the next row of the block, or the next
-        // mode's first row in it
-        auto next = [&](int& g, int& i) {
-            const int jb = i & ~(kSpecRows - 1);
-            if (i + 1 < min(jb + kSpecRows, nT)) {
-                ++i;
-            } else {
-                ++g;
-                i = jb;
+        // (li, ki) of row i + 1 from those of row i
+        auto step_row = [&](int& li, int& ki) {
+            if (++ki == K) {
+                ki = 0;
+                ++li;
             }
         };
-        for (int e0 = 2 * tid; e0 < total; e0 += 2 * 256 * kTriBatch) {
+        for (int p0 = tid; p0 < nPairs; p0 += 256 * kTriBatch) {
             float4 v[kTriBatch];
             int gq[kTriBatch], iq[kTriBatch];
 #pragma unroll
             for (int q = 0; q < kTriBatch; ++q) {
-                const int e = e0 + 512 * q;
-                const int a = locate(e, gq[q], iq[q]);
-                v[q] = ldf4(specR, e < total ? 2 * a : -1);  // entries past the tile read as 0, unused
+                const int p = p0 + 256 * q;
+                const int a = pair(p, gq[q], iq[q]);
+                v[q] = ldf4(specR, p < nPairs ? 2 * a : -1);  // pairs past the tile read as 0, unused
             }
 #pragma unroll
-            for (int q = 0; q < kTriBatch; ++q)
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const int e = e0 + 512 * q + h;
-                    if (e < total) {
-                        if (h) next(gq[q], iq[q]);
-                        int li, ki;
-                        split(iq[q], li, ki);
-                        zs(gq[q], ki, li) = h ? make_float2(v[q].z, v[q].w) : make_float2(v[q].x, v[q].y);
+            for (int q = 0; q < kTriBatch; ++q) {
+                if (p0 + 256 * q < nPairs && iq[q] < nT) {
+                    int li, ki;
+                    split(iq[q], li, ki);
+                    zs(gq[q], ki, li) = make_float2(v[q].x, v[q].y);
+                    if (iq[q] + 1 < nT) {
+                        step_row(li, ki);
+                        zs(gq[q], ki, li) = make_float2(v[q].z, v[q].w);
                     }
                 }
+            }
         }
         __syncthreads();
         const double h2 = s.invH2;
@@ -1477,21 +1471,18 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
         }
         __syncthreads();
         if (p64) return;  // the solution is already stored (uniform over the block)
-        // write-back in spectrum order, pairs of entries as float4 where both
-        // lie in the tile
+        // write-back in spectrum order, pairs as float4 where both rows are real
         float* __restrict__ spw = s.spec;
-        auto staged = [&](int g, int i) -> float2 {
+        for (int p = tid; p < nPairs; p += 256) {
+            int g, i;
+            const int a = pair(p, g, i);
+            if (i >= nT) continue;
             int li, ki;
             split(i, li, ki);
-            return zs(g, ki, li);
-        };
-        for (int e = 2 * tid; e < total; e += 512) {
-            int g, i;
-            const int a = locate(e, g, i);
-            const float2 x = staged(g, i);
-            if (e + 1 < total) {
-                next(g, i);
-                const float2 y = staged(g, i);
+            const float2 x = zs(g, ki, li);
+            if (i + 1 < nT) {
+                step_row(li, ki);
+                const float2 y = zs(g, ki, li);
                 *reinterpret_cast<float4*>(spw + 2 * (size_t)a) = make_float4(x.x, x.y, y.x, y.y);
             } else {
                 *reinterpret_cast<float2*>(spw + 2 * (size_t)a) = x;
@@ -1593,7 +1584,7 @@ __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restri
         C* zbuf = zlds + nP;                                        // Rp rows of M
         C* scratch = zbuf + (size_t)(Rp + grp) * M;                 // this lane group's row
         const Rsrc specR = kP64 ? make_rsrc(s.state[parity ^ 1][0], 4 * M * nT)
-                                : make_rsrc(s.spec, 2 * M * nT);
+                                : make_rsrc(s.spec, 2 * M * spec_rows(nT));
         stage_twiddles(tw, s, tid, nthr);
         __syncthreads();
         // ceil(2^32 / Rp) (Rp = 1 splits nothing); idx / Rp is exact for idx < 2^16
@@ -1611,8 +1602,8 @@ __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restri
                     load_spec(specR, n * nT + j, Pn[q]);
                     load_spec(specR, (M - n) * nT + j, Pm[q]);
                 } else {
-                    load_spec(specR, (int)spec_index(n, j, M, nT), Pn[q]);
-                    load_spec(specR, (int)spec_index(M - n, j, M, nT), Pm[q]);
+                    load_spec(specR, (int)spec_index(n, j, M), Pn[q]);
+                    load_spec(specR, (int)spec_index(M - n, j, M), Pm[q]);
                 }
             }
 #pragma unroll

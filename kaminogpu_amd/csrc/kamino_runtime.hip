@@ -192,7 +192,8 @@ int check_precision(int precision, const Plan& plan) {
     return KAM_OK;
 }
 
-size_t sim_floats(int nT, int nP) { return (size_t)7 * nT * nP + 64; }
+// three fields x two copies, the padded spectrum, one watchdog block
+size_t sim_floats(int nT, int nP) { return (size_t)6 * nT * nP + (size_t)nP * spec_rows(nT) + 64; }
 
 // ---------------------------------------------------------------------------
 // Per-shape tables
@@ -401,7 +402,7 @@ void fill_desc(SimDesc& d, const Plan& p, const Tables& t, const kam_sim_desc& s
     for (int b = 0; b < 2; ++b)
         for (int f = 0; f < 3; ++f) d.state[b][f] = mem + (size_t)(3 * b + f) * N;
     d.spec = mem + 6 * N;
-    d.status = reinterpret_cast<int*>(mem + 7 * N);  // callers may point it elsewhere
+    d.status = reinterpret_cast<int*>(mem + 6 * N + (size_t)p.nP * spec_rows(p.nT));  // callers may point it elsewhere
     d.rowf = t.rowf;
     d.rowd = t.rowd;
     d.twN = t.twN;
@@ -1006,7 +1007,7 @@ int kam_sim_get_spectrum(kam_sim* s, float* spec) {
     if (!s || !spec) return set_err(KAM_EINVAL, "null argument");
     KAM_HIP(hipSetDevice(s->sd.device));
     const int M = s->plan.M, nT = s->plan.nT;
-    const size_t n = (size_t)M * nT * 2;
+    const size_t n = (size_t)M * spec_rows(nT) * 2;
     std::vector<float> blocked(n);
     KAM_HIP(hipMemcpyAsync(blocked.data(), s->desc.spec, n * sizeof(float), hipMemcpyDeviceToHost, s->stream));
     KAM_HIP(hipStreamSynchronize(s->stream));
@@ -1014,7 +1015,7 @@ int kam_sim_get_spectrum(kam_sim* s, float* spec) {
     // gets it mode-major
     for (int m = 0; m < M; ++m)
         for (int j = 0; j < nT; ++j) {
-            const size_t src = 2 * spec_index(m, j, M, nT), dst = 2 * ((size_t)m * nT + j);
+            const size_t src = 2 * spec_index(m, j, M), dst = 2 * ((size_t)m * nT + j);
             spec[dst] = blocked[src];
             spec[dst + 1] = blocked[src + 1];
         }
