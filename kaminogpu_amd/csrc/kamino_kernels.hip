@@ -1184,20 +1184,22 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
             }
         }
         // The tile's entries in spectrum order (spec_index): block by block of
-        // four rows, in each the tile's modes as runs of four rows.  Pair p
-        // of the tile (two entries, one float4) is mode row0 + g, rows i and
-        // i + 1 of block b:  b = p / (2 rows), w = p - 2 rows b, g = w / 2,
-        // i = 4 b + 2 (w % 2), at float2 offset 4 b M + 4 row0 + 2 w (even, so
-        // float4-aligned).  Rows past nTheta are padding and are skipped.
+        // kSpecRows rows, in each the tile's modes as runs of kSpecRows rows.
+        // Pair p of the tile (two entries, one float4) is mode row0 + g, rows
+        // i and i + 1 of block b:  b = p / (kHalf rows), w = p - kHalf rows b,
+        // g = w / kHalf, i = kSpecRows b + 2 (w % kHalf), at float2 offset
+        // kSpecRows (b M + row0) + 2 w (even, so float4-aligned).  Rows past
+        // nTheta are padding and are skipped.
         const Rsrc specR = make_rsrc(s.spec, 2 * s.M * spec_rows(nT));
-        const int pairN = 2 * tile.rows, nPairs = tile.rows * (spec_rows(nT) >> 1), Ms = s.M;
+        constexpr int kHalf = kSpecRows / 2;  // pairs per mode in a block
+        const int pairN = kHalf * tile.rows, nPairs = tile.rows * (spec_rows(nT) / 2), Ms = s.M;
         const unsigned magicB = 0xffffffffu / (unsigned)pairN + 1u;  // p / pairN, p < 2^16, pairN >= 2
         auto pair = [&](int p, int& g, int& i) -> int {
             const int b = (int)__umulhi((unsigned)p, magicB);
             const int w = p - b * pairN;
-            g = w >> 1;
-            i = 4 * b + ((w & 1) << 1);
-            return 4 * b * Ms + 4 * tile.row0 + 2 * w;
+            g = w / kHalf;
+            i = kSpecRows * b + 2 * (w % kHalf);
+            return kSpecRows * b * Ms + kSpecRows * tile.row0 + 2 * w;
         };
         // (li, ki) of row i + 1 from those of row i
         auto step_row = [&](int& li, int& ki) {
