@@ -856,7 +856,14 @@ __device__ __forceinline__ void wave_pass(const C* __restrict__ src, C* __restri
         for (int q = 0; q < RADIX; ++q) {
             C v = in[q * nb];
             if (q > 0) {
-                C w = tw[q * j * twScale];
+                C w;
+                if constexpr (std::is_same<C, double2>::value) {
+                    // FP64 twiddles are held for n < M only: w(n + M) = -w(n)
+                    const int t = q * j * twScale;
+                    w = t < M ? tw[t] : mkc(-tw[t - M].x, -tw[t - M].y);
+                } else {
+                    w = tw[q * j * twScale];
+                }
                 if (DIR > 0) w.y = -w.y;
                 v = cmul(v, w);
             }
@@ -1550,17 +1557,18 @@ __device__ __forceinline__ void load_spec(Rsrc r, int idx, double2& v) {
     v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, idx * 16, 0, 0));
 }
 
-// Stages the nPhi FP64 twiddles of s into LDS, one double2 (16 bytes) each.
+// Stages the first M = nPhi / 2 FP64 twiddles of s into LDS, one double2
+// (16 bytes) each; the transform takes the others as w(n + M) = -w(n).
 __device__ __forceinline__ void stage_twiddles(double2* twl, const SimDesc& s, int tid, int nthr) {
     const Rsrc tr = make_rsrc(reinterpret_cast<const float*>(s.twN64), 4 * s.nPhi);
     float4* dst = reinterpret_cast<float4*>(twl);
-    for (int i0 = tid; i0 < s.nPhi; i0 += 4 * nthr) {
+    for (int i0 = tid; i0 < s.M; i0 += 4 * nthr) {
         float4 v[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = ldf4(tr, 4 * (i0 + q * nthr));
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-            if (i0 + q * nthr < s.nPhi) dst[i0 + q * nthr] = v[q];
+            if (i0 + q * nthr < s.M) dst[i0 + q * nthr] = v[q];
     }
 }
 
@@ -1582,8 +1590,8 @@ __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restri
         const int gsh = s.fftShift, gs = 1 << gsh;  // lanes per row (a power of two)
         // kFftBlock threads: kFftBlock / gs lane groups (a power of two)
         const int ngsh = kFftBlockShift - gsh, grp = tid >> gsh, ngrp = 1 << ngsh, lane = tid & (gs - 1);
-        C* tw = zlds;                                               // nPhi twiddles
-        C* zbuf = zlds + nP;                                        // Rp rows of M
+        C* tw = zlds;                                               // nPhi twiddles (M in FP64)
+        C* zbuf = zlds + (kP64 ? M : nP);                           // Rp rows of M
         C* scratch = zbuf + (size_t)(Rp + grp) * M;                 // this lane group's row
         const Rsrc specR = kP64 ? make_rsrc(s.state[parity ^ 1][0], 4 * M * nT)
                                 : make_rsrc(s.spec, 2 * M * spec_rows(nT));

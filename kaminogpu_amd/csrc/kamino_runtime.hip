@@ -74,11 +74,11 @@ static int fft_group_host(int M) {
     while (g < 64 && 4 * g < M) g *= 2;
     return g;
 }
-// LDS of an FFT tile: the nPhi = 2M twiddles, R + 1 rows of M complex and the
-// lane groups' scratch rows; FP32 complex (8 bytes), or FP64 (16 bytes) for the
-// inverse transform of the FP64 pressure
+// LDS of an FFT tile: the nPhi = 2M twiddles (M in FP64), R + 1 rows of M
+// complex and the lane groups' scratch rows; FP32 complex (8 bytes), or FP64
+// (16 bytes) for the inverse transform of the FP64 pressure
 static size_t fft_lds(int R, int M, bool p64 = false) {
-    return (size_t)(2 + R + 1 + 256 / fft_group_host(M)) * M * (p64 ? 16 : 8);
+    return (size_t)((p64 ? 1 : 2) + R + 1 + 256 / fft_group_host(M)) * M * (p64 ? 16 : 8);
 }
 static constexpr size_t kLdsMax = 160 * 1024;
 

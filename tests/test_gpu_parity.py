@@ -106,7 +106,7 @@ def test_projection_parity(nt, precision):
         assert dm <= gate_p32(nt)
 
 
-@pytest.mark.parametrize("nt", [16, 48, 256])
+@pytest.mark.parametrize("nt", [16, 45, 48, 256])
 def test_spectrum_and_solve(nt):
     """d-hat from the GPU row FFT against the oracle's shifted spectrum, and the
     GPU per-mode solve against an FP64 dense solve of the GPU's own d-hat."""
