@@ -139,8 +139,9 @@ int kam_sim_benchmark(kam_sim *sim, int nsteps, int per_kernel, double *step_ms,
 
 /* Solves the tridiagonal systems of the current spectrum in place, exactly as
  * KAM_PHASE_TRI does (call it after KAM_PHASE_DIVFFT), and reports the
- * relative Poisson residual ||T_n p_n - d_n|| / ||d_n|| of each mode's FP64
- * solution before it is rounded to FP32 storage: per_mode[M = n_phi/2]
+ * relative Poisson residual ||T_n p_n - d_n|| / ||d_n|| of each mode's
+ * solution (the FP32 elimination of the FP32 pressure, the FP64 one of the
+ * FP64 pressure), formed in FP64 before the store: per_mode[M = n_phi/2]
  * (optional; slot 0 is the Nyquist mode), their maximum and root mean square.
  * A diagnostic of the solve (no counterpart in the reference, whose crKernel
  * solves in FP32, tdm.cu:3-96). */

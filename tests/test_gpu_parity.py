@@ -149,10 +149,11 @@ def test_spectrum_and_solve(nt):
 
 
 @pytest.mark.parametrize("nt", [32, 48, 256, 1024])
-def test_poisson_residual_of_the_fp64_solve(nt):
+def test_poisson_residual_gate(nt):
     """Gate (i) of the divergence criterion: per mode, ||T p - d|| / ||d|| of the
-    solver's FP64 solution is at most 1e-6 (it sits near FP64 rounding); the
-    reporting launch leaves the same solution as the plain tridiagonal phase."""
+    solver's solution (FP32 elimination with subtraction-free pivots), formed
+    in FP64, is at most 1e-6; the reporting launch leaves the same solution as
+    the plain tridiagonal phase."""
     dt = 0.5 / nt
     state = seeded_state(nt, L=min(16, nt // 4), seed=11, dens=1)
     with K.Sim(nt, dt) as s:
