@@ -304,7 +304,7 @@ class Sim:
 
     def poisson_residual(self):
         """Solves the current spectrum in place (as PHASE_TRI) and returns the
-        per-mode relative residual of the FP64 solution, float64 [M]."""
+        per-mode relative residual of the solution, formed in FP64, float64 [M]."""
         r = np.empty(self.n_phi // 2, np.float64)
         mx, rms = C.c_double(), C.c_double()
         check(lib().kam_sim_poisson_residual(self._h, r.ctypes.data_as(C.POINTER(C.c_double)),
