@@ -1,6 +1,7 @@
 // kamino_devutil.h -- small device helpers shared by the kernel sources:
-// buffer-resource loads (32-bit offsets, hardware range check: reads past
-// the range return 0), the XCD-aware tile mapping and the watchdog flag.
+// buffer-resource loads and stores (32-bit offsets, hardware range check:
+// reads past the range return 0), the XCD-aware tile mapping and the
+// watchdog flag.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,6 +24,12 @@ __device__ __forceinline__ double ldd(Rsrc r, int elem) {
 }
 __device__ __forceinline__ float4 ldf4(Rsrc r, int elem) {
     return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, elem * 4, 0, 0));
+}
+// A store at byte offset vbytes + sbytes: a row walk passes the lane's column
+// in vbytes (loop-invariant) and the row's base in sbytes (a scalar), so the
+// address costs no vector instruction.  Stores past the range are dropped.
+__device__ __forceinline__ void stf(Rsrc r, int vbytes, int sbytes, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, vbytes, sbytes, 0);
 }
 __device__ __forceinline__ float lerpf(float a, float b, float t) { return fmaf(t, b - a, a); }
 

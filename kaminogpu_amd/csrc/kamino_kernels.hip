@@ -282,6 +282,13 @@ __device__ __forceinline__ int opaque(int v) {
     asm volatile("" : "+v"(v));
     return v;
 }
+// The same for a wavefront-uniform value, kept in a scalar register: a row
+// walk's addresses are then a scalar row base plus the lane's column, not
+// loop-carried vector registers.
+__device__ __forceinline__ int opaque_s(int v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
 
 // (int)floorf(x) in one instruction (the compiler selects it only under
 // no-NaN math); x is a departure coordinate, |x| < 2^24.
@@ -378,7 +385,8 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
                                                 const Tile* __restrict__ tiles, int ntiles,
                                                 int parity) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
-    const int tid = threadIdx.x, nthr = blockDim.x;
+    constexpr int nthr = kAdvWaves * 64;  // launch_advect's block size
+    const int tid = threadIdx.x;
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
         const SimDesc& s = sims[tile.sim];
@@ -391,6 +399,8 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         float* __restrict__ oup = s.state[parity ^ 1][F_UPHI];
         float* __restrict__ out = s.state[parity ^ 1][F_UTHETA];
         float* __restrict__ orh = s.state[parity ^ 1][F_RHO];
+        const Rsrc oupR = make_rsrc(oup, N), outR = make_rsrc(out, N - nP), orhR = make_rsrc(orh, N);
+        int* const status = s.status;
         const float* __restrict__ cofPC = s.rowf + RT_COFPHI_C * nT;
         const float* __restrict__ cofPT = s.rowf + RT_COFPHI_T * nT;
         const float hT = 0.5f * s.cofTheta;  // half of theta cells per unit u_theta
@@ -415,9 +425,6 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         st.rlo[K_VPHI] = st.rlo[K_VTHETA] = st.rlo[K_CENTERED] = max(0, 1 - js);
         st.rhi[K_VPHI] = st.rhi[K_CENTERED] = min(SR - 1, nT - 1 - js);
         st.rhi[K_VTHETA] = min(SR - 1, nT - 2 - js);
-        // per wavefront: max |u_phi|, |u_theta| of its staged items and max
-        // half phi coefficient of its node rows (float bits)
-        __shared__ unsigned wmax[3][kAdvWaves];
         // half phi coefficients of node rows j0-1 .. j0+R (u_phi / rho nodes,
         // u_theta nodes) and the geometric coefficient G of those rows, after
         // the staged planes and the advected u block
@@ -427,6 +434,11 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         cofRow[0] = lds + 2 * plane + max(plane, uvN);
         cofRow[1] = cofRow[0] + (R + 2);
         cofRow[2] = cofRow[1] + (R + 2);
+        // per wavefront: max |u_phi|, |u_theta| of its staged items and max
+        // half phi coefficient of its node rows (float bits), after the
+        // coefficients.  The kernel has no static LDS, so the dynamic block
+        // starts at address 0 and LDS offsets fold into the instructions.
+        unsigned(*wmax)[kAdvWaves] = reinterpret_cast<unsigned(*)[kAdvWaves]>(cofRow[2] + (R + 2));
         unsigned mc = 0u;
         if (tid < R + 2) {  // node rows j0-1 .. j0+R, one per thread
             const int j = j0 - 1 + tid;
@@ -541,6 +553,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
 
         if (!(s.flags & SIM_GEOMETRIC)) {
             auto plain = [&](auto F) {
+                bool bad = false;
                 for (int idx = tid; idx < R * W; idx += nthr) {
                     const int r = (int)__umulhi((unsigned)idx, mW), c = idx - r * W;
                     const int j = j0 + r, o = j * nP + c0 + c;
@@ -548,8 +561,9 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
                     if (j < nT - 1) out[o] = node(F, KTheta{}, c, j, 0.5f * cofPT[j]);
                     const float rr = node(F, KRho{}, c, j, 0.5f * cofPC[j]);
                     orh[o] = rr;
-                    flag_bad(s.status, !isfinite(rr));
+                    bad |= !isfinite(rr);
                 }
+                flag_bad(status, bad);
             };
             if (fast)
                 plain(std::true_type{});
@@ -577,19 +591,27 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         // wavefront down.  Other widths split a flat item index; those loops end
         // at wavefront granularity, with no duplicated tail items (occupancy
         // hides the LDS latency).  The mapping never changes a value.
-        const bool colMap = (W == 64 || W == 128);
-        const int cm = tid & (W - 1), rmS = nthr >> (W == 128 ? 7 : 6), rm0 = tid >> (W == 128 ? 7 : 6);
+        // A stride-SW launch holds widths up to SW - 2 kEC, and the column map
+        // serves exactly that width: the row step rmS is then a constant, and
+        // rm0 (a whole wavefront's tile row, kColW being a multiple of 64) is
+        // uniform over the wavefront, so the walks' trip counts are scalar.
+        constexpr int kColW = SW - 2 * kEC, rmS = nthr / kColW;
+        static_assert(kColW % 64 == 0 && nthr % kColW == 0, "a column map row is whole wavefronts");
+        const bool colMap = (W == kColW);
+        const int cm = tid & (kColW - 1), rm0 = __builtin_amdgcn_readfirstlane(tid / kColW);
+        const int colB = (c0 + cm) * 4;  // the lane's column in a row of the grid, in bytes
         const int nUr = uHi - uLo + 1, nVr = vHi - vLo + 1;
         auto advect_all = [&](auto F) {
             if (colMap) {
                 const int dOfs = __mul24(rmS, SW);
                 int ofs = __mul24(j0 + rm0 - js, SW) + cm + kEC;
-                float* o = orh + (j0 + rm0) * nP + c0 + cm;
-                for (int r = rm0; r < R; r += rmS, ofs += dOfs, o += rmS * nP) {
+                bool bad = false;
+                for (int r = rm0; r < R; r += rmS, ofs += dOfs) {
                     const float a = nodeAt(F, KRho{}, cm, j0 + r, cofRow[0][r + 1], ofs);
-                    *o = a;
-                    flag_bad(s.status, !isfinite(a));
+                    stf(orhR, colB, (j0 + r) * nP * 4, a);
+                    bad |= !isfinite(a);
                 }
+                flag_bad(status, bad);
                 __syncthreads();  // the staged rho plane is dead from here on
                 ofs = __mul24(uLo + rm0 - js, SW) + cm + kEC;
                 float* u = U + __mul24(rm0, W2) + cm + 1;
@@ -610,12 +632,14 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
                 }
                 return;
             }
+            bool bad = false;
             for (int idx = tid; idx < nR; idx += nthr) {
                 const int r = (int)__umulhi((unsigned)idx, mW), c = idx - __mul24(r, W), j = j0 + r;
                 const float a = node(F, KRho{}, c, j, cofRow[0][r + 1]);
                 orh[j * nP + c0 + c] = a;
-                flag_bad(s.status, !isfinite(a));
+                bad |= !isfinite(a);
             }
+            flag_bad(status, bad);
             __syncthreads();  // the staged rho plane is dead from here on
             for (int idx = tid; idx < nU; idx += nthr) {
                 const int r = (int)__umulhi((unsigned)idx, mW2), c = idx - __mul24(r, W2) - 1, j = uLo + r;
@@ -667,7 +691,7 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         };
         if (colMap) {
             const int nGr = uHi - j0 + 1;
-            for (int r = rm0; r < nGr; r += rmS) geo(r, cm);
+            for (int r = rm0; r < nGr; r += rmS) geo(opaque_s(r), cm);
             for (int r = nthr - 1 - tid; r < nGr; r += nthr) geo(r, W);  // column W
         } else {
             for (int idx = tid; idx < nG; idx += nthr) {
@@ -684,7 +708,11 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
             if (j0 + r < uHi) out[o] = 0.5f * (VP[x + 1] + VP[x + W1 + 1]);
         };
         if (colMap) {
-            for (int r = rm0; r < R; r += rmS) assign(r, cm);
+            for (int r = rm0; r < R; r += rmS) {
+                const int rr = opaque_s(r), x = rr * W1 + cm, rowB = (j0 + rr) * nP * 4;
+                stf(oupR, colB, rowB, 0.5f * (X[x] + X[x + 1]));
+                if (j0 + rr < uHi) stf(outR, colB, rowB, 0.5f * (VP[x + 1] + VP[x + W1 + 1]));
+            }
         } else {
             for (int idx = tid; idx < nR; idx += nthr) {
                 const int r = (int)__umulhi((unsigned)idx, mW);
