@@ -421,8 +421,12 @@ __global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims
         st.S = lds;
         st.plane = plane;
         st.cmax = SC - 1;
-        // ordinary-branch rows: jt >= 1 and jt < pole row (nT-1; nT-2 for u_theta)
-        st.rlo[K_VPHI] = st.rlo[K_VTHETA] = st.rlo[K_CENTERED] = max(0, 1 - js);
+        // stencil rows that read as the ordinary branch: jt >= 0 and jt < pole
+        // row (nT-1; nT-2 for u_theta).  A top row jt = 0 reached without a
+        // fold (the raw row index is 0, not -1) takes rows 0 and 1 at the
+        // same columns and weights in the global sampler's fold branch too.
+        // With nT >= 4 and SR >= 7 every range is non-empty (rhi > rlo).
+        st.rlo[K_VPHI] = st.rlo[K_VTHETA] = st.rlo[K_CENTERED] = max(0, -js);
         st.rhi[K_VPHI] = st.rhi[K_CENTERED] = min(SR - 1, nT - 1 - js);
         st.rhi[K_VTHETA] = min(SR - 1, nT - 2 - js);
         // half phi coefficients of node rows j0-1 .. j0+R (u_phi / rho nodes,
