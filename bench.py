@@ -45,9 +45,11 @@ def parse():
     ap.add_argument("--large-transform", choices=["auto", "fft", "dft"], default="auto",
                     help="phi transform of the config-2 leg: the autotuner's pick, or forced")
     ap.add_argument("--serve-requests", type=int, default=10000, help="config-4 requests (0 skips)")
-    ap.add_argument("--launch", choices=["graph", "direct"], default="direct",
-                    help="engine steps as direct kernel launches (default: each hipGraph launch "
-                         "boundary idles the GPU ~10 us) or as hipGraph replays")
+    ap.add_argument("--launch", choices=["graph", "direct"], default="graph",
+                    help="engine steps of the batched legs as hipGraph replays or as direct kernel "
+                         "launches (the config-3 leg measures both; the served leg launches directly)")
+    ap.add_argument("--graph-steps", type=int, default=32,
+                    help="steps captured per hipGraph (each graph launch boundary idles the GPU ~10 us)")
     ap.add_argument("--tune-cache", default=None,
                     help="tuning cache: sizes it holds are loaded, the others autotuned and appended")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -75,9 +77,9 @@ def clock_warmup_engine(eng, seconds):
 
     n, t0 = 0, time.perf_counter()
     while time.perf_counter() - t0 < seconds:
-        eng.step(10)
+        eng.step(16)
         eng.sync()
-        n += 10
+        n += 16
     return n
 
 
@@ -120,6 +122,7 @@ def cpu_baseline(nt, dt, seconds):
 
 TUNED = set()
 USE_GRAPH = False  # set from --launch
+GRAPH_STEPS = 32  # set from --graph-steps
 
 
 def tune(K, nt, batch, cache):
@@ -141,7 +144,9 @@ def tune(K, nt, batch, cache):
 def run_uneven(K, steps, cache, sims=64, seed=3):
     """Config 3: 64 spheres with n_theta drawn from {32,...,256}, dt = 0.5/n_theta,
     L = min(16, n_theta/4), density band, stepped together through one engine;
-    tiles of each size from the autotuner."""
+    tiles of each size from the autotuner.  Steps are replayed from hipGraphs of
+    GRAPH_STEPS steps each; the same leg with direct launches is reported beside
+    it."""
     rng = np.random.default_rng(seed)
     sizes = rng.choice([32, 48, 64, 96, 128, 192, 256], sims)
     tiles = {}
@@ -150,21 +155,29 @@ def run_uneven(K, steps, cache, sims=64, seed=3):
         # is launch latency, and its timings pick tiles at random
         tiles[nt], _ = tune(K, nt, max(64, int((sizes == nt).sum())), cache)
     log(f"uneven tiles: {tiles}")
-    with K.Engine(use_graph=USE_GRAPH) as eng:
-        for k, nt in enumerate(sizes):
-            st = K.init_fields(int(nt), min(16, int(nt) // 4), 100 + k, K.DENSITY_BAND)
-            eng.admit(int(nt), 0.5 / nt, *st, steps=10 ** 6)
-        clock_warmup_engine(eng, CLOCK_WARMUP_S)
-        eng.step(10)
-        eng.reset_stats()
-        eng.step(steps)
-        s = eng.stats()
-        kern = eng.profile(10)
-    v = s.cell_steps / (s.gpu_ms * 1e-3)
-    log(f"uneven x{sims}: {v:.3e} cell-steps/s, p50 {s.step_p50_ms:.4f} p99 {s.step_p99_ms:.4f} ms, "
-        f"kernel ms {np.round(kern, 4)}")
+    forms = {}
+    for form in ("graph", "direct"):
+        with K.Engine(use_graph=form == "graph", graph_steps=GRAPH_STEPS) as eng:
+            for k, nt in enumerate(sizes):
+                st = K.init_fields(int(nt), min(16, int(nt) // 4), 100 + k, K.DENSITY_BAND)
+                eng.admit(int(nt), 0.5 / nt, *st, steps=10 ** 6)
+            clock_warmup_engine(eng, CLOCK_WARMUP_S)
+            eng.step(16)
+            eng.reset_stats()
+            eng.step(steps)
+            s = eng.stats()
+            if form == "graph":
+                kern = eng.profile(10)
+        v = s.cell_steps / (s.gpu_ms * 1e-3)
+        forms[form] = {"cell_steps_per_s": v, "ms_per_step": s.gpu_ms / s.steps,
+                       "p50_step_ms": s.step_p50_ms, "p99_step_ms": s.step_p99_ms}
+        log(f"uneven x{sims} ({form}): {v:.3e} cell-steps/s, {s.gpu_ms / s.steps:.4f} ms/step, "
+            f"p50 {s.step_p50_ms:.4f} p99 {s.step_p99_ms:.4f} ms")
+    log(f"uneven kernel ms {np.round(kern, 4)}")
+    g = forms["graph"]
     return {"sims": sims, "cells": int(sum(2 * int(n) ** 2 for n in sizes)), "steps": steps,
-            "cell_steps_per_s": v, "p50_step_ms": s.step_p50_ms, "p99_step_ms": s.step_p99_ms,
+            "launch": f"hipGraph of {GRAPH_STEPS} steps", "cell_steps_per_s": g["cell_steps_per_s"],
+            "p50_step_ms": g["p50_step_ms"], "p99_step_ms": g["p99_step_ms"], "direct_launches": forms["direct"],
             "kernel_ms": dict(zip(["advect", "div_fft", "tridiag", "ifft_grad"], kern.tolist())),
             "tiles": {str(k): t for k, t in tiles.items()}}
 
@@ -178,7 +191,9 @@ def run_served(K, n):
     reqs = serve.make_requests(n, seed=4)
     work = float(np.mean([2 * r.n_theta ** 2 * r.steps for r in reqs]))
     pool = serve.StatePool()
-    with K.Engine(use_graph=USE_GRAPH) as eng:
+    # direct launches: membership changes every few tens of steps, and each
+    # change would re-capture the graphs
+    with K.Engine() as eng:
         cap = serve.capacity(serve.make_requests(128, seed=5), eng, pool)
         rate = 0.8 * cap / work
         log(f"serve: mix capacity {cap:.3e} cell-steps/s -> {rate:.1f} requests/s")
@@ -279,8 +294,9 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
 
 def main():
     a = parse()
-    global USE_GRAPH, CLOCK_WARMUP_S
+    global USE_GRAPH, GRAPH_STEPS, CLOCK_WARMUP_S
     USE_GRAPH = a.launch == "graph"
+    GRAPH_STEPS = a.graph_steps
     CLOCK_WARMUP_S = a.clock_warmup_s
     if a.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has one MI355X; "
@@ -316,7 +332,7 @@ def main():
     states = [init_state(nt, 16, 1 + b, 1) for b in range(min(B, 8))]
 
     def batched_leg(precision):
-        with K.Engine(use_graph=USE_GRAPH) as eng:
+        with K.Engine(use_graph=USE_GRAPH, graph_steps=GRAPH_STEPS) as eng:
             for b in range(B):
                 if eng.admit(nt, dt, *states[b % len(states)], steps=10 ** 6, precision=precision) is None:
                     raise SystemExit("engine refused the batch")
@@ -371,7 +387,8 @@ def main():
         "data": "synthetic (seeded spherical-harmonic streamfunction, density band)",
         "config": {"workload": f"batched: {B} spheres of {nt}x{2 * nt}, dt {dt}, one engine step "
                                f"advances all {B}", "n_theta": nt, "n_phi": 2 * nt, "sims": B,
-                   "tiles": tiles, "launch": a.launch, "clock_warmup_steps": clock_steps},
+                   "tiles": tiles, "launch": a.launch if a.launch == "direct" else
+                   f"hipGraph of {a.graph_steps} steps", "clock_warmup_steps": clock_steps},
         "batched": {"cell_steps_per_s": batched, "p50_step_ms": st.step_p50_ms,
                     "p99_step_ms": st.step_p99_ms, "speedup_vs_plain": batched / plain if plain else None},
         "batched_fp64_pressure": p64,

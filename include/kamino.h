@@ -254,8 +254,12 @@ typedef struct kam_engine_desc {
     int device;
     int max_slots;            /* concurrently active simulations (0 = 4096) */
     size_t hbm_budget_bytes;  /* arena for simulation state (0 = half of free HBM) */
-    int use_graph;            /* 1: replay each step as a hipGraph */
-    int max_inflight;         /* steps queued ahead of the GPU at most (0 = 2) */
+    int use_graph;            /* 1: replay steps from hipGraphs */
+    int max_inflight;         /* launches (steps or graphs) queued ahead of the GPU at most (0 = 2) */
+    int graph_steps;          /* with use_graph: G steps per graph (0 = 1, at most 64).  G = 1
+                                 replays every step from a one-step graph.  G > 1: a run of G steps
+                                 in which no simulation finishes is one graph launch, the other
+                                 steps are direct launches.  Results equal direct launches. */
 } kam_engine_desc;
 
 /* KAM_REQ_FAILED: quarantined because a step produced a non-finite value in

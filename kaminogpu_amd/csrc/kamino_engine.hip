@@ -3,14 +3,18 @@
 // One engine = one device, one stream, one HBM arena.  Active simulations of
 // any supported shape share every step: the device batch holds their
 // descriptors and tile lists, and a step is the four batched launches of
-// DeviceBatch::launch, replayed from a hipGraph captured per step parity.
-// Membership changes (admission, retirement, cancellation) rebuild the tables
-// and drop the graphs.  All device work is ordered on the engine stream, so a
+// DeviceBatch::launch, issued directly or replayed from a hipGraph captured per
+// step parity.  With graph_steps = G > 1, runs of G steps during which no
+// simulation finishes are captured as one graph of 4G launches, so the cost of
+// a graph launch boundary is paid once per G steps; the steps between such runs
+// go out as direct launches.  Membership changes
+// (admission, retirement, cancellation) rebuild the tables and drop the graphs.  All device work is ordered on the engine stream, so a
 // retired simulation's arena block can be handed to the next admission at
 // once: the new upload is queued after the old download.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <deque>
@@ -143,7 +147,8 @@ struct kam_engine {
     int64_t totalSteps = 0;           // never reset
     int parity = 0;
     bool dirty = true;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    // [0][parity]: one step; [1][parity]: graph_steps steps
+    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     // replaced graphs wait here until their last launch has finished
     std::deque<std::pair<hipGraphExec_t, hipEvent_t>> oldGraphs;
     // statistics
@@ -152,6 +157,7 @@ struct kam_engine {
     // it (else the step's interval starts at the previous step's end).
     struct StepEv {
         hipEvent_t start, end;
+        int n;  // steps between the two events
     };
     std::deque<StepEv> stepEvents;
     std::vector<hipEvent_t> eventPool;
@@ -172,8 +178,9 @@ namespace {
 // Retires the current graphs.  A graph exec owns its kernel arguments, so it
 // is destroyed only once an event queued after its last launch has completed.
 int drop_graphs(kam_engine* e) {
-    for (auto& g : e->graph)
-        if (g) {
+    for (auto& gp : e->graph)
+        for (auto& g : gp) {
+            if (!g) continue;
             hipEvent_t ev = nullptr;
             KAM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
             KAM_HIP(hipEventRecord(ev, e->stream));
@@ -253,7 +260,8 @@ int collect(kam_engine* e, bool wait) {
         if (se.start || e->windowOpen) {
             float ms = 0.f;
             KAM_HIP(hipEventElapsedTime(&ms, se.start ? se.start : e->startEvent, ev));
-            e->stepMs.push_back(ms);
+            // a multi-step graph is timed as a whole: each of its steps gets the mean
+            for (int k = 0; k < se.n; ++k) e->stepMs.push_back(ms / se.n);
         }
         if (se.start) e->eventPool.push_back(se.start);
         // the event becomes the start of the next interval
@@ -305,10 +313,11 @@ void drain(kam_engine* e) {
     }
 }
 
-// Every 8 steps: quarantine requests whose watchdog word was set in the last
-// snapshot, then take a new snapshot (copied on the stream, read 8 steps later).
-int watchdog(kam_engine* e) {
-    if (e->totalSteps % 8 != 0) return KAM_OK;
+// Every 8 steps (at the first step boundary at or past each multiple of 8):
+// quarantine requests whose watchdog word was set in the last snapshot, then
+// take a new snapshot (copied on the stream, read 8 or more steps later).
+int watchdog(kam_engine* e, int64_t before) {
+    if (before / 8 == e->totalSteps / 8) return KAM_OK;
     if (e->statusStep >= 0 && hipEventQuery(e->statusEv) == hipSuccess) {
         std::vector<uint64_t> bad;
         for (uint64_t id : e->active) {
@@ -331,7 +340,10 @@ int watchdog(kam_engine* e) {
     return KAM_OK;
 }
 
-int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
+// Queues `n` steps of every active simulation: one step (n = 1), or one replay
+// of the graph of graph_steps steps (n = graph_steps; the caller has checked
+// that no active simulation finishes inside it).
+int one_step(kam_engine* e, hipEvent_t* kernelEvents, int n = 1) {
     // keep at most max_inflight steps queued ahead of the GPU
     const size_t q = e->stepEvents.size(), lim = (size_t)e->ed.max_inflight;
     if (q >= lim) KAM_HIP(hipEventSynchronize(e->stepEvents[q - lim].end));
@@ -348,12 +360,13 @@ int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
         KAM_HIP(hipEventRecord(startEv, e->stream));
     }
     int np = e->parity;
-    if (e->ed.use_graph && !kernelEvents) {
-        hipGraphExec_t& g = e->graph[e->parity];
+    if (e->ed.use_graph && !kernelEvents && (n > 1 || e->ed.graph_steps == 1)) {
+        hipGraphExec_t& g = e->graph[n > 1][e->parity];
         if (!g) {
             hipGraph_t graph = nullptr;
             KAM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            int rc = e->batch.launch(KAM_PHASE_ALL, e->parity, e->stream, &np);
+            int rc = KAM_OK;
+            for (int k = 0; k < n && rc == KAM_OK; ++k) rc = e->batch.launch(KAM_PHASE_ALL, np, e->stream, &np);
             hipError_t ce = hipStreamEndCapture(e->stream, &graph);
             if (rc != KAM_OK) return rc;
             if (ce != hipSuccess) return set_err(KAM_EHIP, "graph capture: %s", hipGetErrorString(ce));
@@ -362,7 +375,8 @@ int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
             if (ce != hipSuccess) return set_err(KAM_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
         }
         KAM_HIP(hipGraphLaunch(g, e->stream));
-        np = e->parity ^ 1;
+        // every launch sequence flips the parity once per step
+        np = (n & 1) ? e->parity ^ 1 : e->parity;
     } else {
         int rc = e->batch.launch(KAM_PHASE_ALL, e->parity, e->stream, &np, kernelEvents);
         if (rc != KAM_OK) return rc;
@@ -371,14 +385,15 @@ int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
     hipEvent_t ev = take_event(e);
     if (!ev) return set_err(KAM_EHIP, "event creation failed");
     KAM_HIP(hipEventRecord(ev, e->stream));
-    e->stepEvents.push_back({startEv, ev});
-    e->steps++;
-    e->cellSteps += e->curCells;
+    e->stepEvents.push_back({startEv, ev, n});
+    e->steps += n;
+    e->cellSteps += n * e->curCells;
     // advance every request; retire the finished ones
     std::vector<uint64_t> finished;
     for (uint64_t id : e->active) {
         Request& r = e->reqs[id];
-        if (++r.done >= r.total) finished.push_back(id);
+        r.done += n;
+        if (r.done >= r.total) finished.push_back(id);
     }
     for (uint64_t id : finished) {
         int rc = retire(e, e->reqs[id], KAM_REQ_DONE);
@@ -387,8 +402,9 @@ int one_step(kam_engine* e, hipEvent_t* kernelEvents) {
     }
     drain(e);
     reap_graphs(e, false);
-    e->totalSteps++;
-    int rc = watchdog(e);
+    const int64_t before = e->totalSteps;
+    e->totalSteps += n;
+    int rc = watchdog(e, before);
     if (rc != KAM_OK) return rc;
     if (e->stepEvents.size() > 1024) return collect(e, false);
     return KAM_OK;
@@ -407,6 +423,11 @@ int kam_engine_create(const kam_engine_desc* d, kam_engine** out) {
     e->ed = *d;
     if (e->ed.max_slots <= 0) e->ed.max_slots = 4096;
     if (e->ed.max_inflight <= 0) e->ed.max_inflight = 2;
+    if (e->ed.graph_steps <= 0) e->ed.graph_steps = 1;
+    if (e->ed.graph_steps > 64) {
+        delete e;
+        return set_err(KAM_EINVAL, "graph_steps %d above 64", d->graph_steps);
+    }
     e->t0 = std::chrono::steady_clock::now();
     size_t budget = d->hbm_budget_bytes;
     if (budget == 0) {
@@ -526,9 +547,21 @@ int kam_engine_admit(kam_engine* e, const kam_sim_desc* d, const float* u_phi,
 int kam_engine_step(kam_engine* e, int nsteps) {
     if (!e || nsteps < 0) return set_err(KAM_EINVAL, "bad arguments");
     KAM_HIP(hipSetDevice(e->ed.device));
-    for (int k = 0; k < nsteps; ++k) {
-        int rc = one_step(e, nullptr);
+    const int G = e->ed.use_graph ? e->ed.graph_steps : 1;
+    for (int k = 0; k < nsteps;) {
+        // a run of G steps goes out as one graph when no simulation finishes inside it
+        int n = 1;
+        if (G > 1 && nsteps - k >= G && !e->active.empty()) {
+            int64_t left = INT64_MAX;
+            for (uint64_t id : e->active) {
+                const Request& r = e->reqs[id];
+                left = std::min(left, r.total - r.done);
+            }
+            if (left >= G) n = G;
+        }
+        int rc = one_step(e, nullptr, n);
         if (rc != KAM_OK) return rc;
+        k += n;
     }
     return KAM_OK;
 }

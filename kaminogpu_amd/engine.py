@@ -18,7 +18,7 @@ KAM_EAGAIN = -4
 
 class EngineDesc(C.Structure):
     _fields_ = [("device", C.c_int), ("max_slots", C.c_int), ("hbm_budget_bytes", C.c_size_t),
-                ("use_graph", C.c_int), ("max_inflight", C.c_int)]
+                ("use_graph", C.c_int), ("max_inflight", C.c_int), ("graph_steps", C.c_int)]
 
 
 class ReqStatus(C.Structure):
@@ -57,17 +57,20 @@ def bind(L):
 
 class Engine:
     """Batched / served stepping of many simulations on one GPU.  Steps go out
-    as direct kernel launches by default; use_graph=True replays each step as
-    a hipGraph instead (same results; each graph launch boundary costs the GPU
-    ~10 us, more than the launches it saves for steps of 0.1 ms and up)."""
+    as direct kernel launches by default; use_graph=True replays them from
+    hipGraphs instead (same results).  Each graph launch boundary costs the GPU
+    ~10 us, so graph_steps=G > 1 captures G steps per graph and replays it
+    whenever no simulation finishes within the next G steps (the steps between
+    such runs are launched directly)."""
 
-    def __init__(self, device=0, max_slots=0, hbm_budget_bytes=0, use_graph=False, max_inflight=0):
+    def __init__(self, device=0, max_slots=0, hbm_budget_bytes=0, use_graph=False, max_inflight=0,
+                 graph_steps=1):
         from . import check, lib
 
         self._lib = lib()
         self._check = check
         d = EngineDesc(int(device), int(max_slots), int(hbm_budget_bytes), int(bool(use_graph)),
-                       int(max_inflight))
+                       int(max_inflight), int(graph_steps))
         h = C.c_void_p()
         check(self._lib.kam_engine_create(C.byref(d), C.byref(h)))
         self._h = h

@@ -96,6 +96,35 @@ def test_graph_and_direct_launches_agree():
             np.testing.assert_array_equal(x, y)
 
 
+def test_multi_step_graphs_equal_direct_launches():
+    """Runs of 8 steps captured as one graph: requests finishing at step counts
+    that are not multiples of 8, a request admitted mid-run and odd call
+    lengths give the same fields, bitwise, as direct launches, and the step
+    accounting is unchanged."""
+    reqs = [make_request(nt, 500 + k) for k, nt in enumerate([64, 45, 96, 128])]
+    steps = [21, 8, 13, 30]
+    late = make_request(75, 520)
+    out, counts = [], []
+    for graph in (True, False):
+        with K.Engine(use_graph=graph, graph_steps=8) as eng:
+            ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=s) for r, s in zip(reqs, steps)]
+            eng.step(11)
+            lid = eng.admit(late["nt"], late["dt"], *late["state"], steps=19)
+            eng.step(9)
+            eng.step(17)
+            eng.step(3)
+            st = eng.stats()
+            counts.append((st.steps, st.cell_steps, st.completed, st.active))
+            got = [eng.fetch(i, r["nt"]) for r, i in zip(reqs, ids)]
+            got.append(eng.fetch(lid, late["nt"]))
+            out.append(got)
+    assert counts[0] == counts[1] and counts[0][2] == 5 and counts[0][3] == 0, counts
+    for a, b in zip(*out):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    log(f"[engine] 8-step graphs = direct launches bitwise; steps {counts[0][0]}")
+
+
 def test_one_batched_step_matches_oracle():
     """One shared step of a batch spanning the serving mix's sizes (odd n_theta,
     the 64-lane tridiagonal class up to 512): each simulation equals its solo
