@@ -67,19 +67,21 @@ def log(msg):
 CLOCK_WARMUP_S = 0.3  # --clock-warmup-s
 
 
-def clock_warmup_engine(eng, seconds):
+def clock_warmup_engine(eng, seconds, chunk=32):
     """Runs untimed engine steps for `seconds` of wall time before a leg's own
     warm-up steps: the GPU raises its clocks over the first ~50 ms of steady
     work (measured on the 128-sphere batch: 5 warm-up steps give 0.39-0.40 ms
     per step, 200 give 0.331), so a short warm-up would time the ramp, not the
-    step.  Returns the number of steps run."""
+    step.  Steps go out `chunk` at a time (the graph length of a graph-replayed
+    leg, so its graphs are captured and first launched here).  Returns the
+    number of steps run."""
     import time
 
     n, t0 = 0, time.perf_counter()
     while time.perf_counter() - t0 < seconds:
-        eng.step(16)
+        eng.step(chunk)
         eng.sync()
-        n += 16
+        n += chunk
     return n
 
 
@@ -161,7 +163,7 @@ def run_uneven(K, steps, cache, sims=64, seed=3):
             for k, nt in enumerate(sizes):
                 st = K.init_fields(int(nt), min(16, int(nt) // 4), 100 + k, K.DENSITY_BAND)
                 eng.admit(int(nt), 0.5 / nt, *st, steps=10 ** 6)
-            clock_warmup_engine(eng, CLOCK_WARMUP_S)
+            clock_warmup_engine(eng, CLOCK_WARMUP_S, GRAPH_STEPS)
             eng.step(16)
             eng.reset_stats()
             eng.step(steps)
@@ -336,7 +338,7 @@ def main():
             for b in range(B):
                 if eng.admit(nt, dt, *states[b % len(states)], steps=10 ** 6, precision=precision) is None:
                     raise SystemExit("engine refused the batch")
-            clock_steps = clock_warmup_engine(eng, CLOCK_WARMUP_S)
+            clock_steps = clock_warmup_engine(eng, CLOCK_WARMUP_S, GRAPH_STEPS)
             eng.step(a.warmup)
             eng.reset_stats()
             eng.step(a.steps)
