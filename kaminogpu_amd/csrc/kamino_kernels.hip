@@ -867,7 +867,7 @@ __device__ __forceinline__ void group_sync(int gs) {
         wave_sync();
 }
 
-template <int RADIX, int DIR, typename C>
+template <int RADIX, int DIR, bool HALF, typename C>
 __device__ __forceinline__ void wave_pass(const C* __restrict__ src, C* __restrict__ dst,
                                           int M, int Ls, int lsShift, unsigned lsMagic,
                                           const C* __restrict__ tw, int twScale, int lane,
@@ -888,14 +888,13 @@ __device__ __forceinline__ void wave_pass(const C* __restrict__ src, C* __restri
         for (int q = 0; q < RADIX; ++q) {
             C v = in[q * nb];
             if (q > 0) {
+                // a half table holds n < M only: w(n + M) = -w(n)
+                const int t = q * j * twScale;
                 C w;
-                if constexpr (std::is_same<C, double2>::value) {
-                    // FP64 twiddles are held for n < M only: w(n + M) = -w(n)
-                    const int t = q * j * twScale;
+                if constexpr (HALF)
                     w = t < M ? tw[t] : mkc(-tw[t - M].x, -tw[t - M].y);
-                } else {
-                    w = tw[q * j * twScale];
-                }
+                else
+                    w = tw[t];
                 if (DIR > 0) w.y = -w.y;
                 v = cmul(v, w);
             }
@@ -910,11 +909,12 @@ __device__ __forceinline__ void wave_pass(const C* __restrict__ src, C* __restri
 
 // Transforms the M complex values at `row` (LDS) in place; `scratch` is the
 // calling lane group's own M-entry LDS row, `tw` the nPhi twiddles
-// exp(-2 pi i n / nPhi) staged in LDS, `lane` the lane within the group of gs
+// exp(-2 pi i n / nPhi) staged in LDS (with HALF only the first M of them,
+// the others being w(n + M) = -w(n)), `lane` the lane within the group of gs
 // lanes.  Every lane of the wavefront (of the block, for a block-wide group)
 // calls it (rowless groups pass active = false) so that the fences and
 // barriers stay uniform.
-template <int DIR, typename C>
+template <int DIR, bool HALF, typename C>
 __device__ void wave_fft(C* row, C* scratch, const C* tw, const SimDesc& s, int lane, int gs,
                          bool active) {
     const int M = s.M;
@@ -928,10 +928,10 @@ __device__ void wave_fft(C* row, C* scratch, const C* tw, const SimDesc& s, int 
         const int lsShift = ((Ls & (Ls - 1)) == 0) ? __builtin_ctz(Ls) : -1;
         if (active) {
             switch (r) {
-                case 2: wave_pass<2, DIR>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
-                case 3: wave_pass<3, DIR>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
-                case 4: wave_pass<4, DIR>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
-                default: wave_pass<5, DIR>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
+                case 2: wave_pass<2, DIR, HALF>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
+                case 3: wave_pass<3, DIR, HALF>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
+                case 4: wave_pass<4, DIR, HALF>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
+                default: wave_pass<5, DIR, HALF>(src, dst, M, Ls, lsShift, lsMagic, tw, twScale, lane, gs); break;
             }
         }
         group_sync(gs);
@@ -957,9 +957,13 @@ constexpr int kGradBatch = 4;  // column pairs per thread in k_ifft_grad
 // Workgroup size of the FFT kernels (their launches use exactly this).
 constexpr int kFftBlockShift = 8, kFftBlock = 1 << kFftBlockShift;
 
-// Stages the nPhi twiddles of s into LDS, two per float4 (nPhi is even).
-__device__ __forceinline__ void stage_twiddles(float2* twl, const SimDesc& s, int tid, int nthr) {
-    const int n4 = s.nPhi >> 1;
+// Stages the nPhi twiddles of s into LDS, two per float4 (nPhi is even), or
+// with `half` the first M = nPhi / 2 of them (M rounded up to even: the table
+// holds nPhi entries).
+__device__ __forceinline__ int tw_slots(int M) { return (M + 1) & ~1; }
+__device__ __forceinline__ void stage_twiddles(float2* twl, const SimDesc& s, int tid, int nthr,
+                                               bool half) {
+    const int n4 = (half ? tw_slots(s.M) : s.nPhi) >> 1;
     const Rsrc tr = make_rsrc(reinterpret_cast<const float*>(s.twN), 2 * s.nPhi);
     float4* dst = reinterpret_cast<float4*>(twl);
     for (int i0 = tid; i0 < n4; i0 += 4 * nthr) {
@@ -981,7 +985,7 @@ __device__ __forceinline__ void stage_twiddles(float2* twl, const SimDesc& s, in
 // This is the reference's cuFFT CUFFT_INVERSE call plus shiftFKernel
 // (:640-656): d-hat_n = (1/nPhi) sum_i d_i e^{-i n i h}.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict__ sims,
+__global__ __launch_bounds__(kFftBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_div_fft(const SimDesc* __restrict__ sims,
                                                  const Tile* __restrict__ tiles, int ntiles,
                                                  int parity) {
     extern __shared__ __attribute__((aligned(16))) float2 zlds[];
@@ -999,8 +1003,8 @@ __global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict
         const float* __restrict__ divS = s.rowf + RT_DIVS * nT;
         const float* __restrict__ divN = s.rowf + RT_DIVN * nT;
         const float* __restrict__ invHs = s.rowf + RT_INVHS * nT;
-        float2* twl = zlds;                                        // nPhi twiddles
-        float2* zbuf = zlds + nP;                                  // R rows of M
+        float2* twl = zlds;                                        // M twiddles
+        float2* zbuf = zlds + tw_slots(M);                         // R rows of M
         float2* scratch = zbuf + (size_t)(R + grp) * M;            // this lane group's row
         const Rsrc upR = make_rsrc(up, nT * nP), utR = make_rsrc(ut, (nT - 1) * nP);
         const int nPass = (R + ngrp - 1) >> ngsh;                  // uniform trip count
@@ -1026,7 +1030,7 @@ __global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict
                     vs[q] = ldf2(utR, oc + i0);
                 }
                 // the twiddles are staged while the first rows are in flight
-                if (it == 0 && kb == 0) stage_twiddles(twl, s, tid, nthr);
+                if (it == 0 && kb == 0) stage_twiddles(twl, s, tid, nthr, true);
 #pragma unroll
                 for (int q = 0; q < kDivBatch; ++q) {
                     const int k = kb + q * gs + lane;
@@ -1037,7 +1041,7 @@ __global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict
             }
             if (it == 0) __syncthreads();  // twiddles visible to every lane group
             group_sync(gs);
-            wave_fft<-1>(row, scratch, twl, s, lane, gs, active);
+            wave_fft<-1, true>(row, scratch, twl, s, lane, gs, active);
         }
         __syncthreads();
         float2* __restrict__ spec = reinterpret_cast<float2*>(s.spec);
@@ -1627,7 +1631,10 @@ __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restri
         C* scratch = zbuf + (size_t)(Rp + grp) * M;                 // this lane group's row
         const Rsrc specR = kP64 ? make_rsrc(s.state[parity ^ 1][0], 4 * M * nT)
                                 : make_rsrc(s.spec, 2 * M * spec_rows(nT));
-        stage_twiddles(tw, s, tid, nthr);
+        if constexpr (kP64)
+            stage_twiddles(tw, s, tid, nthr);
+        else
+            stage_twiddles(tw, s, tid, nthr, false);
         __syncthreads();
         // ceil(2^32 / Rp) (Rp = 1 splits nothing); idx / Rp is exact for idx < 2^16
         const unsigned magicR = (Rp == 1) ? 0u : 0xffffffffu / (unsigned)Rp + 1u;
@@ -1671,7 +1678,7 @@ __global__ __launch_bounds__(kFftBlock) void k_ifft_grad(const SimDesc* __restri
         const int nPass = (Rp + ngrp - 1) >> ngsh;
         for (int it = 0; it < nPass; ++it) {
             const int r = it * ngrp + grp;
-            wave_fft<+1>(zbuf + (size_t)(r < Rp ? r : 0) * M, scratch, (const C*)tw, s, lane, gs, r < Rp);
+            wave_fft<+1, kP64>(zbuf + (size_t)(r < Rp ? r : 0) * M, scratch, (const C*)tw, s, lane, gs, r < Rp);
         }
         __syncthreads();
         const R* p = reinterpret_cast<const R*>(zbuf);  // p[r][i], row stride nP

@@ -92,7 +92,7 @@ struct TileLayout {
     int dftRowOff = 0, nDftRow = 0, dftOff = 0, nDft = 0;  // SIM_DFT rows and MFMA product tiles
     int ntri = 0;
     TriClass tri[12] = {};
-    size_t advLds = 0, fftLds = 0, fftLds64 = 0;
+    size_t advLds = 0, fftLds = 0, fftLds64 = 0, divLds = 0;
     int advStride = 0;  // staged row stride of the advection launch
 };
 

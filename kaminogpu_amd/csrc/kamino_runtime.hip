@@ -74,11 +74,17 @@ static int fft_group_host(int M) {
     while (g < 64 && 4 * g < M) g *= 2;
     return g;
 }
-// LDS of an FFT tile: the nPhi = 2M twiddles (M in FP64), R + 1 rows of M
-// complex and the lane groups' scratch rows; FP32 complex (8 bytes), or FP64
-// (16 bytes) for the inverse transform of the FP64 pressure
+// LDS of an inverse-transform tile: the nPhi = 2M twiddles (M in FP64), R + 1
+// rows of M complex and the lane groups' scratch rows; FP32 complex (8 bytes),
+// or FP64 (16 bytes) for the FP64 pressure
 static size_t fft_lds(int R, int M, bool p64 = false) {
     return (size_t)((p64 ? 1 : 2) + R + 1 + 256 / fft_group_host(M)) * M * (p64 ? 16 : 8);
+}
+// LDS of a forward-transform tile (k_div_fft): the first M twiddles (rounded
+// up to even), R rows of M complex and the scratch rows; 18 KiB at 4 x 256,
+// eight workgroups per CU
+static size_t div_lds(int R, int M) {
+    return ((size_t)((M + 1) & ~1) + (size_t)(R + 256 / fft_group_host(M)) * M) * 8;
 }
 static constexpr size_t kLdsMax = 160 * 1024;
 
@@ -573,6 +579,7 @@ int plan_tiles(SimDesc* sims, const Plan* plans, int n, int* rowScratch, Tile* o
                 L.nFft64 += p64 ? 1 : 0;
             }
             L.fftLds = std::max(L.fftLds, fft_lds(R, p.M));
+            L.divLds = std::max(L.divLds, div_lds(R, p.M));
             if (p64) L.fftLds64 = std::max(L.fftLds64, fft_lds(R, p.M, true));
         }
     L.nFft = cnt - L.fftOff;
@@ -668,7 +675,7 @@ int launch_step(const SimDesc* dSims, const Tile* dTiles, const TileLayout& L, i
     }
     if (phases & KAM_PHASE_DIVFFT) {
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT], st));
-        KAM_HIP(launch_div_fft(dSims, dTiles + L.fftOff, L.nFft, p, L.fftLds, kGridCap, st));
+        KAM_HIP(launch_div_fft(dSims, dTiles + L.fftOff, L.nFft, p, L.divLds, kGridCap, st));
         KAM_HIP(launch_dft_forward(dSims, dTiles + L.dftRowOff, L.nDftRow, dTiles + L.dftOff, L.nDft, p, st));
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT + 1], st));
     }
