@@ -139,7 +139,7 @@ def tune(K, nt, batch, cache):
         TUNED.add(nt)
     tiles = dict(zip(["adv_rows", "adv_cols", "fft_rows"], K.tune_get(nt)))
     # the phi transform the autotuner picked for this size: the row FFT or the MFMA DFT
-    tiles["transform"] = "dft_mfma" if K.tune_get_transform(nt) == K.TRANSFORM_DFT else "fft"
+    tiles["transform"] = {K.TRANSFORM_DFT: "dft_mfma", K.TRANSFORM_FFT8: "fft_w8"}.get(K.tune_get_transform(nt), "fft")
     return tiles, ms
 
 

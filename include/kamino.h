@@ -45,8 +45,11 @@ enum {
 /* ---- phi transforms (kam_tune_set_transform) ----------------------------- */
 enum {
     KAM_TRANSFORM_FFT = 0,  /* the row FFT in LDS (default) */
-    KAM_TRANSFORM_DFT = 1   /* dense DFT products on MFMA: FP16 hi/lo operands, FP32
+    KAM_TRANSFORM_DFT = 1,  /* dense DFT products on MFMA: FP16 hi/lo operands, FP32
                                accumulate (FP32-pressure simulations, n_theta <= 1024) */
+    KAM_TRANSFORM_FFT8 = 2  /* the row FFT with a forward kernel that stages half the
+                               twiddle table and runs eight waves per SIMD; a batch takes
+                               it when all its FFT simulations do.  Same values as FFT. */
 };
 
 /* ---- precision modes ----------------------------------------------------- */
@@ -342,7 +345,7 @@ int kam_tune_set(int n_theta, int adv_rows, int adv_cols, int fft_rows);
 int kam_tune_get(int n_theta, int *adv_rows, int *adv_cols, int *fft_rows);
 
 /* Records the phi transform (KAM_TRANSFORM_*) for simulations of n_theta
- * created or admitted from now on; kam_autotune picks it by timing both.
+ * created or admitted from now on; kam_autotune picks it by timing them.
  * The reference's transform is cuFFT (KaminoCore.cu:763-764, 812-813;
  * plan KaminoSolver.cu:61-66).  The transform never changes results beyond
  * rounding (both are FP32-accurate). */
@@ -350,8 +353,8 @@ int kam_tune_set_transform(int n_theta, int transform);
 int kam_tune_get_transform(int n_theta, int *transform);
 
 /* Times candidate shapes on `batch` synthetic simulations of n_theta (steps
- * timed steps each): advection tiles, FFT tile heights, then the row FFT
- * against the MFMA DFT (n_theta <= 1024).  Records the fastest with
+ * timed steps each): advection tiles, FFT tile heights, then the two row FFT
+ * forms against each other and against the MFMA DFT (n_theta <= 1024).  Records the fastest with
  * kam_tune_set and kam_tune_set_transform, appends a line
  * "gfx950 n_theta batch adv_rows adv_cols fft_rows adv_ms fft_ms transform"
  * to cache_path (may be NULL), and returns the best per-kernel times in

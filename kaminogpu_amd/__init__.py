@@ -39,7 +39,7 @@ ABI_SYMBOLS = [
     "kam_engine_stats", "kam_engine_reset_stats", "kam_engine_profile", "kam_engine_now",
     "kam_sim_step_noalloc", "kam_sim_get_pressure", "kam_tune_set_transform", "kam_tune_get_transform",
 ]
-TRANSFORM_FFT, TRANSFORM_DFT = 0, 1
+TRANSFORM_FFT, TRANSFORM_DFT, TRANSFORM_FFT8 = 0, 1, 2
 NOALLOC_SCRATCH_BYTES = 16 << 20  # KAM_NOALLOC_SCRATCH_BYTES
 
 
@@ -147,7 +147,7 @@ def tune_set(n_theta, adv_rows=0, adv_cols=0, fft_rows=0):
 
 
 def tune_set_transform(n_theta, transform):
-    """Phi transform for simulations of n_theta created from now on (TRANSFORM_FFT / TRANSFORM_DFT)."""
+    """Phi transform for simulations of n_theta created from now on (TRANSFORM_FFT / _FFT8 / _DFT)."""
     check(lib().kam_tune_set_transform(int(n_theta), int(transform)))
 
 

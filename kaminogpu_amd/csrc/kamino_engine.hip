@@ -733,7 +733,8 @@ int kam_tune_set(int n_theta, int adv_rows, int adv_cols, int fft_rows) {
 }
 
 int kam_tune_set_transform(int n_theta, int transform) {
-    if (n_theta < 4 || (transform != KAM_TRANSFORM_FFT && transform != KAM_TRANSFORM_DFT))
+    if (n_theta < 4 || (transform != KAM_TRANSFORM_FFT && transform != KAM_TRANSFORM_DFT &&
+                        transform != KAM_TRANSFORM_FFT8))
         return set_err(KAM_EINVAL, "bad transform entry");
     return tune_set_transform(n_theta, transform);
 }
@@ -817,8 +818,21 @@ int kam_autotune(int device, int n_theta, int batch, int steps, const char* cach
             bestF = fftC[fi];
         }
     }
-    // the phi transform: the row FFT (just timed) against the MFMA DFT
+    // the phi transform: the row FFT (just timed) against its eight-wave form
+    // and the MFMA DFT
     int bestT = KAM_TRANSFORM_FFT;
+    if (rc == KAM_OK) {
+        tune_set(n_theta, bestR, bestW, bestF);
+        tune_set_transform(n_theta, KAM_TRANSFORM_FFT8);
+        e->dirty = true;
+        if ((rc = kam_engine_profile(e, 2, k)) == KAM_OK && (rc = kam_engine_profile(e, steps, k)) == KAM_OK &&
+            k[KC_DIVFFT] + k[KC_IFFT] < best[KC_DIVFFT] + best[KC_IFFT]) {
+            best[KC_DIVFFT] = k[KC_DIVFFT];
+            best[KC_IFFT] = k[KC_IFFT];
+            best[KC_TRI] = k[KC_TRI];
+            bestT = KAM_TRANSFORM_FFT8;
+        }
+    }
     if (rc == KAM_OK && n_theta <= 1024) {
         tune_set(n_theta, bestR, bestW, bestF);
         tune_set_transform(n_theta, KAM_TRANSFORM_DFT);

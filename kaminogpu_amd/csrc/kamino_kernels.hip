@@ -984,10 +984,13 @@ __device__ __forceinline__ void stage_twiddles(float2* twl, const SimDesc& s, in
 // four, mode-major within a block); slot 0 holds (d-hat_0, d-hat_Nyquist).
 // This is the reference's cuFFT CUFFT_INVERSE call plus shiftFKernel
 // (:640-656): d-hat_n = (1/nPhi) sum_i d_i e^{-i n i h}.
+// HALF: only the first M twiddles are staged (w(n + M) = -w(n)), and the
+// launch (k_div_fft8) asks for eight waves per SIMD; else all nPhi twiddles
+// (k_div_fft).  The two give the same values.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kFftBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_div_fft(const SimDesc* __restrict__ sims,
-                                                 const Tile* __restrict__ tiles, int ntiles,
-                                                 int parity) {
+template <bool HALF>
+__device__ __forceinline__ void div_fft_tiles(const SimDesc* __restrict__ sims, const Tile* __restrict__ tiles,
+                                              int ntiles, int parity) {
     extern __shared__ __attribute__((aligned(16))) float2 zlds[];
     const int tid = threadIdx.x, nthr = blockDim.x;
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
@@ -1003,8 +1006,8 @@ __global__ __launch_bounds__(kFftBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
         const float* __restrict__ divS = s.rowf + RT_DIVS * nT;
         const float* __restrict__ divN = s.rowf + RT_DIVN * nT;
         const float* __restrict__ invHs = s.rowf + RT_INVHS * nT;
-        float2* twl = zlds;                                        // M twiddles
-        float2* zbuf = zlds + tw_slots(M);                         // R rows of M
+        float2* twl = zlds;                                        // M or nPhi twiddles
+        float2* zbuf = zlds + (HALF ? tw_slots(M) : nP);           // R rows of M
         float2* scratch = zbuf + (size_t)(R + grp) * M;            // this lane group's row
         const Rsrc upR = make_rsrc(up, nT * nP), utR = make_rsrc(ut, (nT - 1) * nP);
         const int nPass = (R + ngrp - 1) >> ngsh;                  // uniform trip count
@@ -1030,7 +1033,7 @@ __global__ __launch_bounds__(kFftBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
                     vs[q] = ldf2(utR, oc + i0);
                 }
                 // the twiddles are staged while the first rows are in flight
-                if (it == 0 && kb == 0) stage_twiddles(twl, s, tid, nthr, true);
+                if (it == 0 && kb == 0) stage_twiddles(twl, s, tid, nthr, HALF);
 #pragma unroll
                 for (int q = 0; q < kDivBatch; ++q) {
                     const int k = kb + q * gs + lane;
@@ -1041,7 +1044,7 @@ __global__ __launch_bounds__(kFftBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
             }
             if (it == 0) __syncthreads();  // twiddles visible to every lane group
             group_sync(gs);
-            wave_fft<-1, true>(row, scratch, twl, s, lane, gs, active);
+            wave_fft<-1, HALF>(row, scratch, twl, s, lane, gs, active);
         }
         __syncthreads();
         float2* __restrict__ spec = reinterpret_cast<float2*>(s.spec);
@@ -1066,6 +1069,15 @@ __global__ __launch_bounds__(kFftBlock) __attribute__((amdgpu_waves_per_eu(8, 8)
         }
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(kFftBlock) void k_div_fft(const SimDesc* __restrict__ sims,
+                                                 const Tile* __restrict__ tiles, int ntiles, int parity) {
+    div_fft_tiles<false>(sims, tiles, ntiles, parity);
+}
+__global__ __launch_bounds__(kFftBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_div_fft8(
+    const SimDesc* __restrict__ sims, const Tile* __restrict__ tiles, int ntiles, int parity) {
+    div_fft_tiles<true>(sims, tiles, ntiles, parity);
 }
 
 // ---------------------------------------------------------------------------
@@ -1770,10 +1782,10 @@ hipError_t launch_particles(const SimDesc* sims, const Tile* tiles, int ntiles, 
 }
 
 hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
-                          size_t ldsBytes, int gridCap, hipStream_t st) {
+                          size_t ldsBytes, int gridCap, hipStream_t st, bool half) {
     if (ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_div_fft, dim3(grid_for(ntiles, gridCap)), dim3(kFftBlock), ldsBytes, st, sims,
-                       tiles, ntiles, parity);
+    hipLaunchKernelGGL(half ? k_div_fft8 : k_div_fft, dim3(grid_for(ntiles, gridCap)), dim3(kFftBlock),
+                       ldsBytes, st, sims, tiles, ntiles, parity);
     return hipGetLastError();
 }
 

@@ -18,7 +18,7 @@ hipError_t launch_advect(const SimDesc* sims, const Tile* tiles, int ntiles, int
 hipError_t launch_particles(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
                             int gridCap, hipStream_t st);
 hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
-                          size_t ldsBytes, int gridCap, hipStream_t st);
+                          size_t ldsBytes, int gridCap, hipStream_t st, bool half);
 // `parity` names the current state copy: a simulation with the FP64 pressure
 // keeps its solved spectrum over the other one.
 // fp32: run the elimination in FP32 with subtraction-free pivots (the FP32

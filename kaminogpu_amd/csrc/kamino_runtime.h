@@ -35,7 +35,7 @@ struct Plan {
     int triK = 0;      // template row capacity per lane
     size_t advLds = 0; // bytes of LDS of an advection tile (geometric on)
     size_t fftLds = 0; // bytes of LDS of an FFT tile (forward or inverse)
-    int transform = 0; // KAM_TRANSFORM_FFT or KAM_TRANSFORM_DFT (FP32-pressure simulations)
+    int transform = 0; // KAM_TRANSFORM_FFT, _FFT8, or _DFT (FP32-pressure simulations)
 };
 
 // Validates a shape and fills its plan; returns KAM_OK or KAM_EINVAL.
@@ -93,6 +93,7 @@ struct TileLayout {
     int ntri = 0;
     TriClass tri[12] = {};
     size_t advLds = 0, fftLds = 0, fftLds64 = 0, divLds = 0;
+    bool divW8 = false;  // forward transform by k_div_fft8 (half twiddle table, eight waves)
     int advStride = 0;  // staged row stride of the advection launch
 };
 

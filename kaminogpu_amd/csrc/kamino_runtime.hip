@@ -80,11 +80,13 @@ static int fft_group_host(int M) {
 static size_t fft_lds(int R, int M, bool p64 = false) {
     return (size_t)((p64 ? 1 : 2) + R + 1 + 256 / fft_group_host(M)) * M * (p64 ? 16 : 8);
 }
-// LDS of a forward-transform tile (k_div_fft): the first M twiddles (rounded
-// up to even), R rows of M complex and the scratch rows; 18 KiB at 4 x 256,
-// eight workgroups per CU
-static size_t div_lds(int R, int M) {
-    return ((size_t)((M + 1) & ~1) + (size_t)(R + 256 / fft_group_host(M)) * M) * 8;
+// LDS of a forward-transform tile: the nPhi = 2M twiddles (k_div_fft) or the
+// first M of them rounded up to even (k_div_fft8, `half`), R rows of M complex
+// and the scratch rows; with half twiddles 18 KiB at 4 x 256, eight
+// workgroups per CU
+static size_t div_lds(int R, int M, bool half) {
+    const size_t tw = half ? (size_t)((M + 1) & ~1) : (size_t)2 * M;
+    return (tw + (size_t)(R + 256 / fft_group_host(M)) * M) * 8;
 }
 static constexpr size_t kLdsMax = 160 * 1024;
 
@@ -158,6 +160,7 @@ int make_plan(int nT, int nP, Plan& p) {
     const TuneEntry te = tune_get(nT);
     // the MFMA DFT holds nPhi x nPhi matrices: offered up to n_theta = 1024
     if (te.transform == KAM_TRANSFORM_DFT && nT <= 1024) p.transform = KAM_TRANSFORM_DFT;
+    if (te.transform == KAM_TRANSFORM_FFT8) p.transform = KAM_TRANSFORM_FFT8;
     if (te.advCols >= 8 && te.advCols <= kAdvMaxCols) p.advCols = te.advCols < nP ? te.advCols : nP;
     if (te.advRows > 0 || te.fftRows > 0) adjust_plan(p, te.advRows, te.fftRows);
     if (const char* v = std::getenv("KAM_ADV_COLS")) {
@@ -232,10 +235,19 @@ void compute_tables(int nT, double dt, double radius, float* rowf, double* rowd,
         rowf[RT_TRIC * nT + j] = (j == nT - 1) ? 0.f : (float)(h2 + ct);
         rowf[RT_TRIS * nT + j] = (float)rowd[RD_INVSIN2 * nT + j];
     }
+    // exp(-2 pi i n / nPhi); the second half is the exact negation of the
+    // first (w(n + M) = -w(n)), so kernels that stage only the first half
+    // read the same values as those that stage all of it
+    const int M = nP / 2;
     for (int n = 0; n < nP; ++n) {
-        const double a = 2.0 * kPi * n / nP;
-        tw64[2 * n] = std::cos(a);
-        tw64[2 * n + 1] = -std::sin(a);
+        if (n < M) {
+            const double a = 2.0 * kPi * n / nP;
+            tw64[2 * n] = std::cos(a);
+            tw64[2 * n + 1] = -std::sin(a);
+        } else {
+            tw64[2 * n] = -tw64[2 * (n - M)];
+            tw64[2 * n + 1] = -tw64[2 * (n - M) + 1];
+        }
         tw[2 * n] = (float)tw64[2 * n];
         tw[2 * n + 1] = (float)tw64[2 * n + 1];
     }
@@ -566,6 +578,9 @@ int plan_tiles(SimDesc* sims, const Plan* plans, int n, int* rowScratch, Tile* o
     // FP64 pressure, whose inverse transform is a launch of its own (FP64 rows
     // in LDS: their tiles hold as many rows as fit)
     L.fftOff = cnt;
+    // the eight-wave forward kernel when every FFT simulation asks for it
+    bool anyFft = false, allW8 = true;
+    size_t divFull = 0, divHalf = 0;
     for (int pass = 0; pass < 2; ++pass)
         for (int s = 0; s < n; ++s) {
             const Plan& p = plans[s];
@@ -579,9 +594,14 @@ int plan_tiles(SimDesc* sims, const Plan* plans, int n, int* rowScratch, Tile* o
                 L.nFft64 += p64 ? 1 : 0;
             }
             L.fftLds = std::max(L.fftLds, fft_lds(R, p.M));
-            L.divLds = std::max(L.divLds, div_lds(R, p.M));
+            divFull = std::max(divFull, div_lds(R, p.M, false));
+            divHalf = std::max(divHalf, div_lds(R, p.M, true));
+            anyFft = true;
+            allW8 = allW8 && p.transform == KAM_TRANSFORM_FFT8;
             if (p64) L.fftLds64 = std::max(L.fftLds64, fft_lds(R, p.M, true));
         }
+    L.divW8 = anyFft && allW8;
+    L.divLds = L.divW8 ? divHalf : divFull;
     L.nFft = cnt - L.fftOff;
     L.nFft32 = L.nFft - L.nFft64;
     // SIM_DFT simulations: row runs for the divergence and gradient kernels,
@@ -675,7 +695,7 @@ int launch_step(const SimDesc* dSims, const Tile* dTiles, const TileLayout& L, i
     }
     if (phases & KAM_PHASE_DIVFFT) {
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT], st));
-        KAM_HIP(launch_div_fft(dSims, dTiles + L.fftOff, L.nFft, p, L.divLds, kGridCap, st));
+        KAM_HIP(launch_div_fft(dSims, dTiles + L.fftOff, L.nFft, p, L.divLds, kGridCap, st, L.divW8));
         KAM_HIP(launch_dft_forward(dSims, dTiles + L.dftRowOff, L.nDftRow, dTiles + L.dftOff, L.nDft, p, st));
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_DIVFFT + 1], st));
     }

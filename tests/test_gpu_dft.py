@@ -140,8 +140,34 @@ def test_autotune_records_the_transform(tmp_path):
     ms = K.autotune(48, 16, steps=3, cache_path=cache)
     line = open(cache).read().split()
     log(f"[autotune] 48 -> {line}, kernel ms {np.round(ms, 4)}")
-    assert len(line) == 9 and int(line[8]) in (K.TRANSFORM_FFT, K.TRANSFORM_DFT)
+    assert len(line) == 9 and int(line[8]) in (K.TRANSFORM_FFT, K.TRANSFORM_DFT, K.TRANSFORM_FFT8)
     assert K.tune_get_transform(48) == int(line[8])
     K.tune_set_transform(48, K.TRANSFORM_FFT)
     assert 48 in K.tune_load(cache) and K.tune_get_transform(48) == int(line[8])
     K.tune_set_transform(48, K.TRANSFORM_FFT)
+
+
+def test_eight_wave_forward_transform_equals_the_row_fft():
+    """The forward transform of a batch whose FFT simulations all ask for
+    TRANSFORM_FFT8 (half twiddle table, eight waves per SIMD) gives the same
+    fields, bitwise, as the default kernel: the table's second half is the
+    exact negation of its first.  A batch mixing the two takes the default."""
+    sizes = [45, 64, 256, 512, 1024]
+    reqs = [(nt, seeded_state(nt, L=min(16, nt // 4), seed=80 + k, dens=1)) for k, nt in enumerate(sizes)]
+    out = []
+    for t8 in ([], sizes, sizes[:2]):
+        for nt in t8:
+            K.tune_set_transform(nt, K.TRANSFORM_FFT8)
+        try:
+            with K.Engine() as eng:
+                ids = [eng.admit(nt, 0.5 / nt, *st, steps=3) for nt, st in reqs]
+                eng.step(3)
+                out.append([eng.fetch(i, nt) for (nt, _), i in zip(reqs, ids)])
+        finally:
+            for nt in sizes:
+                K.tune_set_transform(nt, K.TRANSFORM_FFT)
+    for run in out[1:]:
+        for g, a in zip(run, out[0]):
+            for x, y in zip(g, a):
+                np.testing.assert_array_equal(x, y)
+    log(f"[fft8] sizes {sizes}: eight-wave forward transform bitwise equal to the default")
