@@ -1310,7 +1310,13 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
         // dropped: its slot's real part) into (xr, xi), in T: Thomas within
         // the lane, then parallel cyclic reduction of the lanes' last
         // unknowns.
+        // The solve, from the lanes' Thomas passes to the solution's staging
+        // for the write-back, for a row count per lane `cntArg`; a block whose
+        // every lane holds KMAX rows (all its modes active, nTheta = L KMAX)
+        // takes it with the count as a constant, so every row guard folds away.
         T xr[KMAX], xi[KMAX];
+        auto solve = [&](auto cntArg) {
+        const int cnt = cntArg;
         {
             T al[KMAX], ga[KMAX], dr[KMAX], di[KMAX];
 #pragma unroll
@@ -1526,6 +1532,11 @@ __device__ __forceinline__ void tri_tile(const SimDesc* __restrict__ sims, const
                     zs(g, k, l) = make_float2((n == 0) ? 0.f : (float)xr[k], (float)xi[k]);
             }
         }
+        };
+        if (tile.rows == 256 / L && nT == L * KMAX)
+            solve(std::integral_constant<int, KMAX>{});
+        else
+            solve(cnt);
         __syncthreads();
         if (p64) return;  // the solution is already stored (uniform over the block)
         // write-back in spectrum order, pairs as float4 where both rows are real
