@@ -357,6 +357,7 @@ __device__ __forceinline__ unsigned wave_max(unsigned v) {
     for (int m = 32; m >= 1; m >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, m, 64));
     return v;
 }
+constexpr int kAdvWaves = 256 / 64;  // wavefronts of an advection workgroup
 
 // Displacement budgets of the fast path, in cells.  Node columns sit at
 // staged columns kEC-1 .. W+kEC and sample offsets lie within half a cell,
@@ -379,13 +380,12 @@ constexpr float kFastDPhi = 2.4f, kFastDTheta = 1.4f;
 // leaves the CU; the cubic roots reuse the staged block's LDS.
 // Flat item loops split an index into (row, column) with a magic multiply.
 // ---------------------------------------------------------------------------
-template <int SW, int NT>
-__global__ __launch_bounds__(NT) void k_advect(const SimDesc* __restrict__ sims,
-                                               const Tile* __restrict__ tiles, int ntiles,
-                                               int parity) {
+template <int SW>
+__global__ __launch_bounds__(256) void k_advect(const SimDesc* __restrict__ sims,
+                                                const Tile* __restrict__ tiles, int ntiles,
+                                                int parity) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
-    constexpr int nthr = NT;              // launch_advect's block size: 256, or 512 for large tiles
-    constexpr int kAdvWaves = NT / 64;    // its wavefronts
+    constexpr int nthr = kAdvWaves * 64;  // launch_advect's block size
     const int tid = threadIdx.x;
     for (int t = xcd_tile(ntiles); t < ntiles; t = ntiles) {
         const Tile tile = tiles[t];
@@ -1771,19 +1771,14 @@ static inline int grid_for(int ntiles, int cap) {
 }
 
 hipError_t launch_advect(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
-                         int stride, size_t ldsBytes, int gridCap, hipStream_t st, int threads) {
+                         int stride, size_t ldsBytes, int gridCap, hipStream_t st) {
     if (ntiles <= 0) return hipSuccess;
-    const dim3 grid(grid_for(ntiles, gridCap));
-    if (threads == 256 && stride == adv_stride(64))
-        hipLaunchKernelGGL((k_advect<adv_stride(64), 256>), grid, dim3(256), ldsBytes, st, sims, tiles, ntiles, parity);
-    else if (threads == 256 && stride == adv_stride(128))
-        hipLaunchKernelGGL((k_advect<adv_stride(128), 256>), grid, dim3(256), ldsBytes, st, sims, tiles, ntiles,
-                           parity);
-    else if (threads == 512 && stride == adv_stride(64))
-        hipLaunchKernelGGL((k_advect<adv_stride(64), 512>), grid, dim3(512), ldsBytes, st, sims, tiles, ntiles, parity);
-    else if (threads == 512 && stride == adv_stride(128))
-        hipLaunchKernelGGL((k_advect<adv_stride(128), 512>), grid, dim3(512), ldsBytes, st, sims, tiles, ntiles,
-                           parity);
+    if (stride == adv_stride(64))
+        hipLaunchKernelGGL(k_advect<adv_stride(64)>, dim3(grid_for(ntiles, gridCap)), dim3(256),
+                           ldsBytes, st, sims, tiles, ntiles, parity);
+    else if (stride == adv_stride(128))
+        hipLaunchKernelGGL(k_advect<adv_stride(128)>, dim3(grid_for(ntiles, gridCap)), dim3(256),
+                           ldsBytes, st, sims, tiles, ntiles, parity);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -1877,9 +1872,8 @@ static hipError_t allow_lds(const void* fn, size_t bytes) {
 }
 
 hipError_t configure_kernels(size_t bytes) {
-    const void* fns[] = {(const void*)k_advect<adv_stride(64), 256>, (const void*)k_advect<adv_stride(128), 256>,
-                         (const void*)k_advect<adv_stride(64), 512>, (const void*)k_advect<adv_stride(128), 512>,
-                         (const void*)k_div_fft, (const void*)k_div_fft8, (const void*)k_ifft_grad<float2, false>,
+    const void* fns[] = {(const void*)k_advect<adv_stride(64)>, (const void*)k_advect<adv_stride(128)>,
+                         (const void*)k_div_fft, (const void*)k_ifft_grad<float2, false>,
                          (const void*)k_ifft_grad<double2, false>, (const void*)k_ifft_grad<float2, true>,
                          (const void*)k_ifft_grad<double2, true>};
     for (const void* fn : fns) {

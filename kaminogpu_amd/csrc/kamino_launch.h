@@ -12,13 +12,9 @@ namespace kam {
 // row stride of adv_stride(W) floats (W rounded up to 64 or 128, plus halo).
 constexpr int kAdvHaloRows = 3, kAdvHaloCols = 4, kAdvMaxCols = 128, kAdvMaxRows = 64;
 constexpr int adv_stride(int W) { return (W <= 64 ? 64 : 128) + 2 * kAdvHaloCols; }
-// An advection workgroup is 256 threads, or 512 (threads = 512) when its LDS
-// would hold fewer than six 256-thread workgroups per CU: three 512-thread
-// workgroups then keep 24 waves per CU, where 256-thread ones would keep 12-20.
-constexpr int adv_threads(size_t ldsBytes) { return ldsBytes > 160 * 1024 / 6 ? 512 : 256; }
 
 hipError_t launch_advect(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
-                         int stride, size_t ldsBytes, int gridCap, hipStream_t st, int threads);
+                         int stride, size_t ldsBytes, int gridCap, hipStream_t st);
 hipError_t launch_particles(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,
                             int gridCap, hipStream_t st);
 hipError_t launch_div_fft(const SimDesc* sims, const Tile* tiles, int ntiles, int parity,

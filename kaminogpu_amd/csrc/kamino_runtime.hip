@@ -58,11 +58,10 @@ static size_t adv_lds(int R, int W, int SW, bool geometric) {
     // three staged planes; with the geometric update the advected u_phi*,
     // u_theta* and antipodal rows go over the third (rho) plane once the rho
     // nodes are done, spilling past it when larger; then three per-row
-    // coefficients of rows j0-1 .. j0+R and three maxima per wavefront (up to
-    // eight wavefronts)
+    // coefficients of rows j0-1 .. j0+R and three maxima per wavefront
     const size_t plane = (size_t)(R + 2 * kAdvHaloRows) * SW;
     const size_t uv = geometric ? (size_t)(R + 1) * (W + 2) + (R + 2) * (W + 1) + 2 * (W + 1) : 0;
-    return (2 * plane + std::max(plane, uv) + 3 * (size_t)(R + 2) + 3 * (512 / 64)) * sizeof(float);
+    return (2 * plane + std::max(plane, uv) + 3 * (size_t)(R + 2) + 3 * (256 / 64)) * sizeof(float);
 }
 // Lanes per row in the row FFT (a quarter of M, 8..64, power of two); a
 // 256-thread block holds 256 / group lane groups, each with a scratch row.
@@ -689,8 +688,7 @@ int launch_step(const SimDesc* dSims, const Tile* dTiles, const TileLayout& L, i
     }
     if (phases & KAM_PHASE_ADVECT) {
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_ADVECT], st));
-        KAM_HIP(launch_advect(dSims, dTiles + L.advOff, L.nAdv, p, L.advStride, L.advLds, kGridCap, st,
-                              adv_threads(L.advLds)));
+        KAM_HIP(launch_advect(dSims, dTiles + L.advOff, L.nAdv, p, L.advStride, L.advLds, kGridCap, st));
         KAM_HIP(launch_particles(dSims, dTiles + L.partOff, L.nPart, p, kGridCap, st));
         if (ev) KAM_HIP(hipEventRecord(ev[2 * KC_ADVECT + 1], st));
         p ^= 1;

@@ -83,32 +83,6 @@ def test_mixed_tile_widths_equal_alone_bitwise():
     log("[engine] batch mixing 128-, 64- and 32-column tiles bitwise equal to alone")
 
 
-def test_512_thread_advection_tiles_equal_256_thread_ones():
-    """Tiles of 12 or 16 rows x 128 columns (LDS for fewer than six 256-thread
-    workgroups per CU) go out as 512-thread workgroups; every simulation's
-    fields equal those of 8 x 128 tiles on 256 threads, bitwise, including an
-    odd n_theta (scalar staging) and the polar bands."""
-    sizes = [256, 192, 75]
-    reqs = [make_request(nt, 400 + k) for k, nt in enumerate(sizes)]
-    out = []
-    try:
-        for R, W in ((8, 128), (16, 128), (12, 128), (16, 64)):
-            for nt in sizes:
-                K.tune_set(nt, R, W, 4)
-            with K.Engine() as eng:
-                ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=3) for r in reqs]
-                eng.step(3)
-                out.append([eng.fetch(i, r["nt"]) for r, i in zip(reqs, ids)])
-    finally:
-        for nt in sizes:
-            K.tune_set(nt, 0, 0, 0)
-    for run in out[1:]:
-        for g, a in zip(run, out[0]):
-            for x, y in zip(g, a):
-                np.testing.assert_array_equal(x, y)
-    log("[engine] 12x128 / 16x128 tiles on 512 threads bitwise equal to 8x128 on 256")
-
-
 def test_graph_and_direct_launches_agree():
     reqs = [make_request(nt, 7 + k) for k, nt in enumerate([64, 96, 64])]
     out = []
