@@ -193,6 +193,19 @@ def test_serving_mix_continuous_admission(use_graph):
     log(f"[serve] 10 requests through 3 slots in {t} shared steps, all equal to solo runs")
 
 
+def test_graph_steps_bounds():
+    """graph_steps above 64 is refused at creation; 0 means one step per graph."""
+    with pytest.raises(K.KaminoError, match="graph_steps"):
+        K.Engine(use_graph=True, graph_steps=65)
+    r = make_request(32, 9)
+    with K.Engine(use_graph=True, graph_steps=0) as eng:
+        i = eng.admit(r["nt"], r["dt"], *r["state"], steps=2)
+        eng.step(2)
+        got = eng.fetch(i, r["nt"])
+    for a, b in zip(got, alone(r["nt"], r["dt"], r["state"], 2)):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_hbm_budget_admission_and_cancel():
     nt = 64
     r = make_request(nt, 1)
