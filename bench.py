@@ -50,6 +50,9 @@ def parse():
                          "launches (the config-3 leg measures both; the served leg launches directly)")
     ap.add_argument("--graph-steps", type=int, default=32,
                     help="steps captured per hipGraph (each graph launch boundary idles the GPU ~10 us)")
+    ap.add_argument("--groups", type=int, default=3,
+                    help="concurrent engine groups, each on a stream of its own (kernels of different "
+                         "groups overlap)")
     ap.add_argument("--tune-cache", default=None,
                     help="tuning cache: sizes it holds are loaded, the others autotuned and appended")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -125,6 +128,7 @@ def cpu_baseline(nt, dt, seconds):
 TUNED = set()
 USE_GRAPH = False  # set from --launch
 GRAPH_STEPS = 32  # set from --graph-steps
+GROUPS = 3  # set from --groups
 
 
 def tune(K, nt, batch, cache):
@@ -147,8 +151,9 @@ def run_uneven(K, steps, cache, sims=64, seed=3):
     """Config 3: 64 spheres with n_theta drawn from {32,...,256}, dt = 0.5/n_theta,
     L = min(16, n_theta/4), density band, stepped together through one engine;
     tiles of each size from the autotuner.  Steps are replayed from hipGraphs of
-    GRAPH_STEPS steps each; the same leg with direct launches is reported beside
-    it."""
+    GRAPH_STEPS steps each in GROUPS concurrent groups; the same leg with direct
+    launches in one group is reported beside it, and the per-kernel times are
+    those of one group (one launch per kernel covering the whole batch)."""
     rng = np.random.default_rng(seed)
     sizes = rng.choice([32, 48, 64, 96, 128, 192, 256], sims)
     tiles = {}
@@ -159,7 +164,8 @@ def run_uneven(K, steps, cache, sims=64, seed=3):
     log(f"uneven tiles: {tiles}")
     forms = {}
     for form in ("graph", "direct"):
-        with K.Engine(use_graph=form == "graph", graph_steps=GRAPH_STEPS) as eng:
+        graph = form == "graph"
+        with K.Engine(use_graph=graph, graph_steps=GRAPH_STEPS, groups=GROUPS if graph else 1) as eng:
             for k, nt in enumerate(sizes):
                 st = K.init_fields(int(nt), min(16, int(nt) // 4), 100 + k, K.DENSITY_BAND)
                 eng.admit(int(nt), 0.5 / nt, *st, steps=10 ** 6)
@@ -168,7 +174,7 @@ def run_uneven(K, steps, cache, sims=64, seed=3):
             eng.reset_stats()
             eng.step(steps)
             s = eng.stats()
-            if form == "graph":
+            if not graph:
                 kern = eng.profile(10)
         v = s.cell_steps / (s.gpu_ms * 1e-3)
         forms[form] = {"cell_steps_per_s": v, "ms_per_step": s.gpu_ms / s.steps,
@@ -178,7 +184,7 @@ def run_uneven(K, steps, cache, sims=64, seed=3):
     log(f"uneven kernel ms {np.round(kern, 4)}")
     g = forms["graph"]
     return {"sims": sims, "cells": int(sum(2 * int(n) ** 2 for n in sizes)), "steps": steps,
-            "launch": f"hipGraph of {GRAPH_STEPS} steps", "cell_steps_per_s": g["cell_steps_per_s"],
+            "launch": f"hipGraph of {GRAPH_STEPS} steps", "groups": GROUPS, "cell_steps_per_s": g["cell_steps_per_s"],
             "p50_step_ms": g["p50_step_ms"], "p99_step_ms": g["p99_step_ms"], "direct_launches": forms["direct"],
             "kernel_ms": dict(zip(["advect", "div_fft", "tridiag", "ifft_grad"], kern.tolist())),
             "tiles": {str(k): t for k, t in tiles.items()}}
@@ -195,7 +201,7 @@ def run_served(K, n):
     pool = serve.StatePool()
     # direct launches: membership changes every few tens of steps, and each
     # change would re-capture the graphs
-    with K.Engine() as eng:
+    with K.Engine(groups=GROUPS) as eng:
         cap = serve.capacity(serve.make_requests(128, seed=5), eng, pool)
         rate = 0.8 * cap / work
         log(f"serve: mix capacity {cap:.3e} cell-steps/s -> {rate:.1f} requests/s")
@@ -296,9 +302,10 @@ def copy_bandwidth(nbytes=1 << 30, reps=10):
 
 def main():
     a = parse()
-    global USE_GRAPH, GRAPH_STEPS, CLOCK_WARMUP_S
+    global USE_GRAPH, GRAPH_STEPS, GROUPS, CLOCK_WARMUP_S
     USE_GRAPH = a.launch == "graph"
     GRAPH_STEPS = a.graph_steps
+    GROUPS = a.groups
     CLOCK_WARMUP_S = a.clock_warmup_s
     if a.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has one MI355X; "
@@ -334,19 +341,27 @@ def main():
     states = [init_state(nt, 16, 1 + b, 1) for b in range(min(B, 8))]
 
     def batched_leg(precision):
-        with K.Engine(use_graph=USE_GRAPH, graph_steps=GRAPH_STEPS) as eng:
+        def admitted(groups):
+            eng = K.Engine(use_graph=USE_GRAPH, graph_steps=GRAPH_STEPS, groups=groups)
             for b in range(B):
                 if eng.admit(nt, dt, *states[b % len(states)], steps=10 ** 6, precision=precision) is None:
                     raise SystemExit("engine refused the batch")
+            return eng
+        with admitted(GROUPS) as eng:
             clock_steps = clock_warmup_engine(eng, CLOCK_WARMUP_S, GRAPH_STEPS)
             eng.step(a.warmup)
             eng.reset_stats()
             eng.step(a.steps)
             st = eng.stats()
+        # per-kernel times of the whole batch as one launch per kernel (one
+        # group, HIP events around each launch): the roofline's kernel, as the
+        # rocprof kernel trace of `--groups 1` records it
+        with admitted(1) as eng:
+            clock2 = clock_warmup_engine(eng, 0.1, GRAPH_STEPS)
             kern_ms = eng.profile(a.profile_steps)
-        return st, kern_ms, clock_steps
+        return st, kern_ms, (clock_steps, clock2)
 
-    st, kern_ms, clock_steps = batched_leg(K.PREC_FP32)
+    st, kern_ms, (clock_steps, clock2) = batched_leg(K.PREC_FP32)
     ms_per_step = st.gpu_ms / st.steps
     batched = st.cell_steps / (st.gpu_ms * 1e-3)
     log(f"batched x{B}: {batched:.3e} cell-steps/s, {ms_per_step:.4f} ms/step, "
@@ -390,7 +405,8 @@ def main():
         "config": {"workload": f"batched: {B} spheres of {nt}x{2 * nt}, dt {dt}, one engine step "
                                f"advances all {B}", "n_theta": nt, "n_phi": 2 * nt, "sims": B,
                    "tiles": tiles, "launch": a.launch if a.launch == "direct" else
-                   f"hipGraph of {a.graph_steps} steps", "clock_warmup_steps": clock_steps},
+                   f"hipGraph of {a.graph_steps} steps", "groups": a.groups, "clock_warmup_steps": clock_steps,
+                   "profile_clock_warmup_steps": clock2},
         "batched": {"cell_steps_per_s": batched, "p50_step_ms": st.step_p50_ms,
                     "p99_step_ms": st.step_p99_ms, "speedup_vs_plain": batched / plain if plain else None},
         "batched_fp64_pressure": p64,

@@ -263,6 +263,9 @@ typedef struct kam_engine_desc {
                                  replays every step from a one-step graph.  G > 1: a run of G steps
                                  in which no simulation finishes is one graph launch, the other
                                  steps are direct launches.  Results equal direct launches. */
+    int groups;               /* concurrent groups (0 = 1, at most 4): the active simulations are split
+                                 into groups of about equal cell count, each stepped on a stream of
+                                 its own so that their kernels overlap; results are unchanged */
 } kam_engine_desc;
 
 /* KAM_REQ_FAILED: quarantined because a step produced a non-finite value in

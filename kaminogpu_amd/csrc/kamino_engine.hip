@@ -11,6 +11,15 @@
 // (admission, retirement, cancellation) rebuild the tables and drop the graphs.  All device work is ordered on the engine stream, so a
 // retired simulation's arena block can be handed to the next admission at
 // once: the new upload is queued after the old download.
+//
+// With groups = G > 1 the active simulations are split into G groups of about
+// equal cell count, each with its own batch, stream and graphs.  A step queues
+// every group's launches on its own stream and then makes the engine stream
+// wait for all of them, so the groups' kernels overlap on the device (one
+// group's launch tails and ramps filled by the others' work) while uploads,
+// result copies, the watchdog snapshot and step timing stay on the engine
+// stream, ordered after every group's step.  After a membership change the
+// group streams wait for the engine stream's uploads before their next launch.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -113,6 +122,7 @@ struct Request {
     size_t off = 0, bytes = 0;
     int64_t total = 0, done = 0;
     int slot = -1;            // index of its watchdog word
+    int group = 0;            // the group whose batch holds it
     int64_t admitStep = 0;    // engine step count at admission
     int state = KAM_REQ_ACTIVE;
     bool copying = false;  // retired; final copy to the host still in flight
@@ -126,12 +136,24 @@ struct Request {
 
 using namespace kam;
 
+constexpr int kMaxGroups = 4;
+
 struct kam_engine {
     kam_engine_desc ed{};
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the engine stream (group 0's stream)
     Arena arena;
     PinnedPool pinned;
-    DeviceBatch batch;
+    struct Group {
+        DeviceBatch batch;
+        hipStream_t stream = nullptr;  // group 0: the engine stream
+        // [0][parity]: one step; [1][parity]: graph_steps steps
+        hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+        hipEvent_t done = nullptr;     // groups > 0: recorded after each step's launches
+        int64_t cells = 0;
+    };
+    Group groups[kMaxGroups];
+    int ngroups = 1;
+    hipEvent_t built = nullptr;     // engine stream, after a rebuild's uploads (groups > 1)
     std::unordered_map<uint64_t, Request> reqs;
     std::vector<uint64_t> active;     // batch order
     std::deque<uint64_t> draining;    // retired, final copy in flight (retirement order)
@@ -147,8 +169,6 @@ struct kam_engine {
     int64_t totalSteps = 0;           // never reset
     int parity = 0;
     bool dirty = true;
-    // [0][parity]: one step; [1][parity]: graph_steps steps
-    hipGraphExec_t graph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     // replaced graphs wait here until their last launch has finished
     std::deque<std::pair<hipGraphExec_t, hipEvent_t>> oldGraphs;
     // statistics
@@ -178,15 +198,16 @@ namespace {
 // Retires the current graphs.  A graph exec owns its kernel arguments, so it
 // is destroyed only once an event queued after its last launch has completed.
 int drop_graphs(kam_engine* e) {
-    for (auto& gp : e->graph)
-        for (auto& g : gp) {
+    for (int gi = 0; gi < e->ngroups; ++gi)
+        for (auto& gp : e->groups[gi].graph)
+            for (auto& g : gp) {
             if (!g) continue;
             hipEvent_t ev = nullptr;
             KAM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
             KAM_HIP(hipEventRecord(ev, e->stream));
             e->oldGraphs.emplace_back(g, ev);
             g = nullptr;
-        }
+            }
     return KAM_OK;
 }
 
@@ -202,15 +223,14 @@ void reap_graphs(kam_engine* e, bool wait) {
 }
 
 int rebuild(kam_engine* e) {
-    std::vector<SimDesc> sims;
-    std::vector<Plan> plans;
-    sims.reserve(e->active.size());
-    plans.reserve(e->active.size());
+    std::vector<SimDesc> sims[kMaxGroups];
+    std::vector<Plan> plans[kMaxGroups];
     // tallest simulations first: their tiles start first
     std::stable_sort(e->active.begin(), e->active.end(), [e](uint64_t a, uint64_t b) {
         return e->reqs[a].plan.nT > e->reqs[b].plan.nT;
     });
     e->curCells = 0;
+    for (int g = 0; g < e->ngroups; ++g) e->groups[g].cells = 0;
     for (uint64_t id : e->active) {
         Request& r = e->reqs[id];
         // pick up tuning-table changes; tile shapes never change values
@@ -223,14 +243,28 @@ int rebuild(kam_engine* e) {
             int rc = attach_transform(r.desc, fresh, r.sd.precision, e->ed.device);
             if (rc != KAM_OK) return rc;
         }
-        sims.push_back(r.desc);
-        plans.push_back(r.plan);
-        e->curCells += (int64_t)r.plan.nT * r.plan.nP;
+        // the group with the fewest cells so far (tallest first: balanced)
+        int g = 0;
+        for (int k = 1; k < e->ngroups; ++k)
+            if (e->groups[k].cells < e->groups[g].cells) g = k;
+        r.group = g;
+        sims[g].push_back(r.desc);
+        plans[g].push_back(r.plan);
+        const int64_t cells = (int64_t)r.plan.nT * r.plan.nP;
+        e->groups[g].cells += cells;
+        e->curCells += cells;
     }
     int rc = drop_graphs(e);
     if (rc != KAM_OK) return rc;
-    rc = e->batch.build(sims, plans, e->stream);
-    if (rc != KAM_OK) return rc;
+    // uploads go out on the engine stream, after the last step's join
+    for (int g = 0; g < e->ngroups; ++g) {
+        rc = e->groups[g].batch.build(sims[g], plans[g], e->stream);
+        if (rc != KAM_OK) return rc;
+    }
+    if (e->ngroups > 1) {
+        KAM_HIP(hipEventRecord(e->built, e->stream));
+        for (int g = 1; g < e->ngroups; ++g) KAM_HIP(hipStreamWaitEvent(e->groups[g].stream, e->built, 0));
+    }
     e->dirty = false;
     return KAM_OK;
 }
@@ -359,27 +393,41 @@ int one_step(kam_engine* e, hipEvent_t* kernelEvents, int n = 1) {
         if (!startEv) return set_err(KAM_EHIP, "event creation failed");
         KAM_HIP(hipEventRecord(startEv, e->stream));
     }
-    int np = e->parity;
-    if (e->ed.use_graph && !kernelEvents && (n > 1 || e->ed.graph_steps == 1)) {
-        hipGraphExec_t& g = e->graph[n > 1][e->parity];
-        if (!g) {
-            hipGraph_t graph = nullptr;
-            KAM_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-            int rc = KAM_OK;
-            for (int k = 0; k < n && rc == KAM_OK; ++k) rc = e->batch.launch(KAM_PHASE_ALL, np, e->stream, &np);
-            hipError_t ce = hipStreamEndCapture(e->stream, &graph);
-            if (rc != KAM_OK) return rc;
-            if (ce != hipSuccess) return set_err(KAM_EHIP, "graph capture: %s", hipGetErrorString(ce));
-            ce = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (ce != hipSuccess) return set_err(KAM_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
+    // every launch sequence flips the parity once per step
+    const int np = (n & 1) ? e->parity ^ 1 : e->parity;
+    const bool useGraph = e->ed.use_graph && !kernelEvents && (n > 1 || e->ed.graph_steps == 1);
+    for (int gi = 0; gi < e->ngroups; ++gi) {
+        kam_engine::Group& G = e->groups[gi];
+        if (G.batch.nsims() == 0) continue;
+        // profiling: every group on the engine stream, each with its own events
+        hipStream_t st = kernelEvents ? e->stream : G.stream;
+        if (useGraph) {
+            hipGraphExec_t& g = G.graph[n > 1][e->parity];
+            if (!g) {
+                hipGraph_t graph = nullptr;
+                KAM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                int rc = KAM_OK, p = e->parity;
+                for (int k = 0; k < n && rc == KAM_OK; ++k) rc = G.batch.launch(KAM_PHASE_ALL, p, st, &p);
+                hipError_t ce = hipStreamEndCapture(st, &graph);
+                if (rc != KAM_OK) return rc;
+                if (ce != hipSuccess) return set_err(KAM_EHIP, "graph capture: %s", hipGetErrorString(ce));
+                ce = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ce != hipSuccess) return set_err(KAM_EHIP, "graph instantiate: %s", hipGetErrorString(ce));
+            }
+            KAM_HIP(hipGraphLaunch(g, st));
+        } else {
+            int p = e->parity;
+            for (int k = 0; k < n; ++k) {
+                int rc = G.batch.launch(KAM_PHASE_ALL, p, st, &p,
+                                        kernelEvents ? kernelEvents + 2 * KC_COUNT * gi : nullptr);
+                if (rc != KAM_OK) return rc;
+            }
         }
-        KAM_HIP(hipGraphLaunch(g, e->stream));
-        // every launch sequence flips the parity once per step
-        np = (n & 1) ? e->parity ^ 1 : e->parity;
-    } else {
-        int rc = e->batch.launch(KAM_PHASE_ALL, e->parity, e->stream, &np, kernelEvents);
-        if (rc != KAM_OK) return rc;
+        if (st != e->stream) {
+            KAM_HIP(hipEventRecord(G.done, st));
+            KAM_HIP(hipStreamWaitEvent(e->stream, G.done, 0));
+        }
     }
     e->parity = np;
     hipEvent_t ev = take_event(e);
@@ -428,6 +476,12 @@ int kam_engine_create(const kam_engine_desc* d, kam_engine** out) {
         delete e;
         return set_err(KAM_EINVAL, "graph_steps %d above 64", d->graph_steps);
     }
+    if (e->ed.groups <= 0) e->ed.groups = 1;
+    if (e->ed.groups > kMaxGroups) {
+        delete e;
+        return set_err(KAM_EINVAL, "groups %d above %d", d->groups, kMaxGroups);
+    }
+    e->ngroups = e->ed.groups;
     e->t0 = std::chrono::steady_clock::now();
     size_t budget = d->hbm_budget_bytes;
     if (budget == 0) {
@@ -442,6 +496,17 @@ int kam_engine_create(const kam_engine_desc* d, kam_engine** out) {
         hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return rc != KAM_OK ? rc : set_err(KAM_EHIP, "stream creation failed");
+    }
+    e->groups[0].stream = e->stream;
+    for (int g = 1; g < e->ngroups; ++g)
+        if (hipStreamCreateWithFlags(&e->groups[g].stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&e->groups[g].done, hipEventDisableTiming) != hipSuccess) {
+            kam_engine_destroy(e);
+            return set_err(KAM_EHIP, "group stream creation failed");
+        }
+    if (e->ngroups > 1 && hipEventCreateWithFlags(&e->built, hipEventDisableTiming) != hipSuccess) {
+        kam_engine_destroy(e);
+        return set_err(KAM_EHIP, "event creation failed");
     }
     const size_t sb = (size_t)e->ed.max_slots * sizeof(int);
     if (hipMalloc(&e->dStatus, sb) != hipSuccess || hipHostMalloc(&e->hStatus, sb, hipHostMallocDefault) != hipSuccess ||
@@ -458,6 +523,8 @@ int kam_engine_create(const kam_engine_desc* d, kam_engine** out) {
 int kam_engine_destroy(kam_engine* e) {
     if (!e) return KAM_OK;
     (void)hipSetDevice(e->ed.device);
+    for (int g = 1; g < e->ngroups; ++g)
+        if (e->groups[g].stream) (void)hipStreamSynchronize(e->groups[g].stream);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     drop_graphs(e);
     reap_graphs(e, true);
@@ -476,6 +543,11 @@ int kam_engine_destroy(kam_engine* e) {
     if (e->statusEv) (void)hipEventDestroy(e->statusEv);
     if (e->dStatus) (void)hipFree(e->dStatus);
     if (e->hStatus) (void)hipHostFree(e->hStatus);
+    for (int g = 1; g < e->ngroups; ++g) {
+        if (e->groups[g].done) (void)hipEventDestroy(e->groups[g].done);
+        if (e->groups[g].stream) (void)hipStreamDestroy(e->groups[g].stream);
+    }
+    if (e->built) (void)hipEventDestroy(e->built);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return KAM_OK;
@@ -569,7 +641,9 @@ int kam_engine_step(kam_engine* e, int nsteps) {
 int kam_engine_profile(kam_engine* e, int nsteps, double* kernel_ms) {
     if (!e || nsteps <= 0) return set_err(KAM_EINVAL, "bad arguments");
     KAM_HIP(hipSetDevice(e->ed.device));
-    hipEvent_t kev[2 * KC_COUNT];
+    // per group, an event before and after each kernel phase; the groups run
+    // one after another on the engine stream and their times add up
+    hipEvent_t kev[kMaxGroups * 2 * KC_COUNT];
     for (auto& ev : kev) KAM_HIP(hipEventCreate(&ev));
     double sum[KC_COUNT] = {0, 0, 0, 0};
     int rc = KAM_OK;
@@ -577,9 +651,13 @@ int kam_engine_profile(kam_engine* e, int nsteps, double* kernel_ms) {
         rc = one_step(e, kev);
         if (rc != KAM_OK) break;
         KAM_HIP(hipStreamSynchronize(e->stream));
-        for (int c = 0; c < KC_COUNT; ++c) {
-            float ms = 0.f;
-            if (hipEventElapsedTime(&ms, kev[2 * c], kev[2 * c + 1]) == hipSuccess) sum[c] += ms;
+        for (int g = 0; g < e->ngroups; ++g) {
+            if (e->groups[g].batch.nsims() == 0) continue;
+            const hipEvent_t* ev = kev + 2 * KC_COUNT * g;
+            for (int c = 0; c < KC_COUNT; ++c) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, ev[2 * c], ev[2 * c + 1]) == hipSuccess) sum[c] += ms;
+            }
         }
     }
     for (auto& ev : kev) (void)hipEventDestroy(ev);

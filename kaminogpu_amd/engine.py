@@ -18,7 +18,8 @@ KAM_EAGAIN = -4
 
 class EngineDesc(C.Structure):
     _fields_ = [("device", C.c_int), ("max_slots", C.c_int), ("hbm_budget_bytes", C.c_size_t),
-                ("use_graph", C.c_int), ("max_inflight", C.c_int), ("graph_steps", C.c_int)]
+                ("use_graph", C.c_int), ("max_inflight", C.c_int), ("graph_steps", C.c_int),
+                ("groups", C.c_int)]
 
 
 class ReqStatus(C.Structure):
@@ -61,16 +62,17 @@ class Engine:
     hipGraphs instead (same results).  Each graph launch boundary costs the GPU
     ~10 us, so graph_steps=G > 1 captures G steps per graph and replays it
     whenever no simulation finishes within the next G steps (the steps between
-    such runs are launched directly)."""
+    such runs are launched directly).  groups=G > 1 splits the simulations into
+    G groups stepped on streams of their own, whose kernels overlap."""
 
     def __init__(self, device=0, max_slots=0, hbm_budget_bytes=0, use_graph=False, max_inflight=0,
-                 graph_steps=1):
+                 graph_steps=1, groups=1):
         from . import check, lib
 
         self._lib = lib()
         self._check = check
         d = EngineDesc(int(device), int(max_slots), int(hbm_budget_bytes), int(bool(use_graph)),
-                       int(max_inflight), int(graph_steps))
+                       int(max_inflight), int(graph_steps), int(groups))
         h = C.c_void_p()
         check(self._lib.kam_engine_create(C.byref(d), C.byref(h)))
         self._h = h

@@ -125,6 +125,36 @@ def test_multi_step_graphs_equal_direct_launches():
     log(f"[engine] 8-step graphs = direct launches bitwise; steps {counts[0][0]}")
 
 
+@pytest.mark.parametrize("groups,graph", [(2, True), (2, False), (3, True)])
+def test_concurrent_groups_equal_one_group(groups, graph):
+    """Simulations split into groups stepped on streams of their own give the
+    same fields, bitwise, and the same step accounting as one group: sizes of
+    every kind, retirements at step counts that are not multiples of the graph
+    length, and admissions mid-run (which rebuild every group)."""
+    sizes = [256, 64, 45, 96, 128, 75, 192]
+    steps = [21, 8, 13, 30, 17, 9, 26]
+    reqs = [make_request(nt, 600 + k) for k, nt in enumerate(sizes)]
+    late = [make_request(48, 620), make_request(160, 621)]
+    out, counts = [], []
+    for g in (1, groups):
+        with K.Engine(use_graph=graph, graph_steps=8, groups=g) as eng:
+            ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=s) for r, s in zip(reqs, steps)]
+            eng.step(11)
+            lids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=19) for r in late]
+            eng.step(9)
+            eng.step(17)
+            st = eng.stats()
+            counts.append((st.steps, st.cell_steps, st.completed, st.active))
+            got = [eng.fetch(i, r["nt"]) for r, i in zip(reqs, ids)]
+            got += [eng.fetch(i, r["nt"]) for r, i in zip(late, lids)]
+            out.append(got)
+    assert counts[0] == counts[1] and counts[0][2] == 9 and counts[0][3] == 0, counts
+    for a, b in zip(*out):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+    log(f"[engine] {groups} concurrent groups (graph={graph}) bitwise equal to one group")
+
+
 def test_one_batched_step_matches_oracle():
     """One shared step of a batch spanning the serving mix's sizes (odd n_theta,
     the 64-lane tridiagonal class up to 512): each simulation equals its solo
