@@ -21,7 +21,7 @@ Config 1: one 256x512 sphere, dt 0.002, all 1000 steps.  Config 2: one
 1024x2048 sphere, dt 0.5/1024, the first 100 of its 500 steps (the FP64
 oracle's 500 steps take minutes on the host).  Config 3: all 64 uneven
 spheres over all 200 steps, stepped together by the engine from 32-step
-hipGraphs, as the benchmark runs them.
+hipGraphs in three concurrent groups, as the benchmark runs them.
 """
 import numpy as np
 import pytest
@@ -79,7 +79,7 @@ def test_trajectory_uneven_batch_config3():
     sizes = [int(n) for n in rng.choice([32, 48, 64, 96, 128, 192, 256], 64)]
     steps = 200
     states = [seeded_state(nt, L=min(16, nt // 4), seed=100 + k, dens=1) for k, nt in enumerate(sizes)]
-    with K.Engine(use_graph=True, graph_steps=32) as eng:
+    with K.Engine(use_graph=True, graph_steps=32, groups=3) as eng:
         ids = [eng.admit(nt, 0.5 / nt, *st, steps=steps) for nt, st in zip(sizes, states)]
         assert all(i is not None for i in ids)
         eng.step(steps)
