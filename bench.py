@@ -220,7 +220,9 @@ def run_large(K, steps, cache, nt=1024, transform="auto"):
     """Config 2: one 1024x2048 sphere, dt = 0.5/nTheta, seeded flow with L = 32,
     stepped alone (plain launches, FP32 fields and pressure); tiles autotuned
     on the sphere itself.  transform "fft" or "dft" overrides the autotuner's
-    choice of the phi transform (the MFMA DFT products, for profiling them)."""
+    choice of the phi transform (the MFMA DFT products, for profiling them).
+    The same sphere as the only member of an engine, replayed from hipGraphs of
+    GRAPH_STEPS steps, is reported beside it."""
     dt = 0.5 / nt
     tiles, _ = tune(K, nt, 1, cache)
     if transform != "auto":
@@ -235,9 +237,20 @@ def run_large(K, steps, cache, nt=1024, transform="auto"):
         _, _, kern = s.benchmark(10, per_kernel=True)
     v = nt * 2 * nt * steps / (total * 1e-3)
     log(f"large {nt}x{2 * nt}: {v:.3e} cell-steps/s, {total / steps:.4f} ms/step, kernel ms {np.round(kern, 4)}")
+    with K.Engine(use_graph=True, graph_steps=GRAPH_STEPS) as eng:
+        eng.admit(nt, dt, *init_state(nt, 32, 2, 1), steps=10 ** 7)
+        clock_warmup_engine(eng, CLOCK_WARMUP_S, GRAPH_STEPS)
+        eng.reset_stats()
+        eng.step(steps)
+        sg = eng.stats()
+    vg = sg.cell_steps / (sg.gpu_ms * 1e-3)
+    log(f"large {nt}x{2 * nt} in an engine, {GRAPH_STEPS}-step graphs: {vg:.3e} cell-steps/s, "
+        f"{sg.gpu_ms / sg.steps:.4f} ms/step, p99 {sg.step_p99_ms:.4f}")
     return {"n_theta": nt, "steps": steps, "cell_steps_per_s": v, "ms_per_step": total / steps,
             "p99_step_ms": float(np.percentile(per, 99)), "tiles": tiles,
-            "kernel_ms": dict(zip(KERNELS, [float(x) for x in kern]))}
+            "kernel_ms": dict(zip(KERNELS, [float(x) for x in kern])),
+            "engine_graph": {"launch": f"hipGraph of {GRAPH_STEPS} steps", "cell_steps_per_s": vg,
+                             "ms_per_step": sg.gpu_ms / sg.steps, "p99_step_ms": sg.step_p99_ms}}
 
 
 def profile_counters(kernel, nt, batch):
