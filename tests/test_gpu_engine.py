@@ -236,6 +236,22 @@ def test_graph_steps_bounds():
         np.testing.assert_array_equal(a, b)
 
 
+def test_groups_bounds_and_empty_groups():
+    """groups above 4 is refused; with more groups than simulations the empty
+    groups launch nothing and the results equal solo runs."""
+    with pytest.raises(K.KaminoError, match="groups"):
+        K.Engine(groups=5)
+    reqs = [make_request(nt, 30 + k) for k, nt in enumerate([64, 45])]
+    for graph in (True, False):
+        with K.Engine(use_graph=graph, graph_steps=4, groups=4) as eng:
+            ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=6) for r in reqs]
+            eng.step(6)
+            got = [eng.fetch(i, r["nt"]) for r, i in zip(reqs, ids)]
+        for r, g in zip(reqs, got):
+            for a, b in zip(g, alone(r["nt"], r["dt"], r["state"], 6)):
+                np.testing.assert_array_equal(a, b)
+
+
 def test_hbm_budget_admission_and_cancel():
     nt = 64
     r = make_request(nt, 1)
