@@ -153,7 +153,7 @@ struct kam_engine {
     };
     Group groups[kMaxGroups];
     int ngroups = 1;
-    hipEvent_t built = nullptr;     // engine stream, after a rebuild's uploads (groups > 1)
+    hipEvent_t built = nullptr;     // engine stream, after a rebuild's uploads or a profiled step (groups > 1)
     std::unordered_map<uint64_t, Request> reqs;
     std::vector<uint64_t> active;     // batch order
     std::deque<uint64_t> draining;    // retired, final copy in flight (retirement order)
@@ -428,6 +428,12 @@ int one_step(kam_engine* e, hipEvent_t* kernelEvents, int n = 1) {
             KAM_HIP(hipEventRecord(G.done, st));
             KAM_HIP(hipStreamWaitEvent(e->stream, G.done, 0));
         }
+    }
+    if (kernelEvents && e->ngroups > 1) {
+        // the groups' states advanced on the engine stream: their own streams
+        // wait for it before their next step
+        KAM_HIP(hipEventRecord(e->built, e->stream));
+        for (int g = 1; g < e->ngroups; ++g) KAM_HIP(hipStreamWaitEvent(e->groups[g].stream, e->built, 0));
     }
     e->parity = np;
     hipEvent_t ev = take_event(e);

@@ -236,6 +236,24 @@ def test_graph_steps_bounds():
         np.testing.assert_array_equal(a, b)
 
 
+def test_groups_steps_around_a_profile():
+    """Steps timed per kernel (kam_engine_profile, every group on the engine
+    stream) between ordinary steps of three groups: the fields equal those of
+    one group doing the same, bitwise."""
+    reqs = [make_request(nt, 40 + k) for k, nt in enumerate([128, 64, 96, 45, 75])]
+    out = []
+    for groups in (1, 3):
+        with K.Engine(use_graph=True, graph_steps=4, groups=groups) as eng:
+            ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=20) for r in reqs]
+            eng.step(5)
+            eng.profile(3)
+            eng.step(12)
+            out.append([eng.fetch(i, r["nt"]) for r, i in zip(reqs, ids)])
+    for a, b in zip(*out):
+        for x, y in zip(a, b):
+            np.testing.assert_array_equal(x, y)
+
+
 def test_groups_bounds_and_empty_groups():
     """groups above 4 is refused; with more groups than simulations the empty
     groups launch nothing and the results equal solo runs."""
