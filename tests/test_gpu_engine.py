@@ -129,16 +129,19 @@ def test_multi_step_graphs_equal_direct_launches():
 def test_concurrent_groups_equal_one_group(groups, graph):
     """Simulations split into groups stepped on streams of their own give the
     same fields, bitwise, and the same step accounting as one group: sizes of
-    every kind, retirements at step counts that are not multiples of the graph
-    length, and admissions mid-run (which rebuild every group)."""
+    every kind, both precisions, retirements at step counts that are not
+    multiples of the graph length, and admissions mid-run (which rebuild every
+    group)."""
     sizes = [256, 64, 45, 96, 128, 75, 192]
     steps = [21, 8, 13, 30, 17, 9, 26]
+    precs = [K.PREC_FP64_PRESSURE if k in (1, 4) else K.PREC_FP32 for k in range(len(sizes))]
     reqs = [make_request(nt, 600 + k) for k, nt in enumerate(sizes)]
     late = [make_request(48, 620), make_request(160, 621)]
     out, counts = [], []
     for g in (1, groups):
         with K.Engine(use_graph=graph, graph_steps=8, groups=g) as eng:
-            ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=s) for r, s in zip(reqs, steps)]
+            ids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=s, precision=p)
+                   for r, s, p in zip(reqs, steps, precs)]
             eng.step(11)
             lids = [eng.admit(r["nt"], r["dt"], *r["state"], steps=19) for r in late]
             eng.step(9)
@@ -181,8 +184,8 @@ def test_one_batched_step_matches_oracle():
             assert errs[2] < TOL_FIELD
 
 
-@pytest.mark.parametrize("use_graph", [False, True])
-def test_serving_mix_continuous_admission(use_graph):
+@pytest.mark.parametrize("use_graph,groups", [(False, 1), (True, 1), (False, 3)])
+def test_serving_mix_continuous_admission(use_graph, groups):
     """Requests arrive while others run; each is admitted as a slot frees up,
     runs its own step count inside shared steps, and equals its solo run
     (with graph replay, every membership change re-captures the step)."""
@@ -196,7 +199,7 @@ def test_serving_mix_continuous_admission(use_graph):
         pending.append(r)
     pending.sort(key=lambda r: r["arrive"])
     done = {}
-    with K.Engine(max_slots=3, use_graph=use_graph) as eng:
+    with K.Engine(max_slots=3, use_graph=use_graph, groups=groups) as eng:
         running = {}
         t = 0
         while pending or running:
