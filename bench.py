@@ -3,8 +3,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
 
-Prints ONE JSON line.  The workload is B simulations of 256x512 (config 1's
-sphere) stepped together by the batching engine; a step is one full time step
+Prints ONE JSON line.  The workload is B simulations of 256x512 stepped together by the batching engine; a step is one full time step
 (advection + geometric + FFT/tridiagonal projection) of all B spheres.  The
 plain form (one sphere, its own kernel sequence, no graph) is measured beside
 it.  Inputs are resident in HBM before the timed region; timing uses HIP
@@ -40,14 +39,14 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--uneven-steps", type=int, default=200, help="config-3 batch steps (0 skips)")
-    ap.add_argument("--large-steps", type=int, default=500, help="config-2 steps at 1024x2048 (0 skips)")
+    ap.add_argument("--uneven-steps", type=int, default=200, help="steps of the uneven 64-sphere batch (0 skips)")
+    ap.add_argument("--large-steps", type=int, default=500, help="steps of the one 1024x2048 sphere (0 skips)")
     ap.add_argument("--large-transform", choices=["auto", "fft", "dft"], default="auto",
-                    help="phi transform of the config-2 leg: the autotuner's pick, or forced")
-    ap.add_argument("--serve-requests", type=int, default=10000, help="config-4 requests (0 skips)")
+                    help="phi transform of the 1024x2048 leg: the autotuner's pick, or forced")
+    ap.add_argument("--serve-requests", type=int, default=10000, help="requests of the served leg (0 skips)")
     ap.add_argument("--launch", choices=["graph", "direct"], default="graph",
                     help="engine steps of the batched legs as hipGraph replays or as direct kernel "
-                         "launches (the config-3 leg measures both; the served leg launches directly)")
+                         "launches (the uneven-batch leg measures both; the served leg launches directly)")
     ap.add_argument("--graph-steps", type=int, default=32,
                     help="steps captured per hipGraph (each graph launch boundary idles the GPU ~10 us)")
     ap.add_argument("--groups", type=int, default=3,
@@ -148,7 +147,7 @@ def tune(K, nt, batch, cache):
 
 
 def run_uneven(K, steps, cache, sims=64, seed=3):
-    """Config 3: 64 spheres with n_theta drawn from {32,...,256}, dt = 0.5/n_theta,
+    """The uneven batch: 64 spheres with n_theta drawn from {32,...,256}, dt = 0.5/n_theta,
     L = min(16, n_theta/4), density band, stepped together through one engine;
     tiles of each size from the autotuner.  Steps are replayed from hipGraphs of
     GRAPH_STEPS steps each in GROUPS concurrent groups; the same leg with direct
@@ -191,7 +190,7 @@ def run_uneven(K, steps, cache, sims=64, seed=3):
 
 
 def run_served(K, n):
-    """Config 4: Poisson arrivals at 80% of the mix's saturated capacity (128
+    """Serving: Poisson arrivals at 80% of the mix's saturated capacity (128
     requests of the same distribution stepped together); n_theta log-uniform
     in [32, 512] (rounded to supported sizes), steps U{50..2000}."""
     from kaminogpu_amd import serve
@@ -217,7 +216,7 @@ def run_served(K, n):
 
 
 def run_large(K, steps, cache, nt=1024, transform="auto"):
-    """Config 2: one 1024x2048 sphere, dt = 0.5/nTheta, seeded flow with L = 32,
+    """One large sphere: 1024x2048, dt = 0.5/nTheta, seeded flow with L = 32,
     stepped alone (plain launches, FP32 fields and pressure); tiles autotuned
     on the sphere itself.  transform "fft" or "dft" overrides the autotuner's
     choice of the phi transform (the MFMA DFT products, for profiling them).

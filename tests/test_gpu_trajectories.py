@@ -1,4 +1,4 @@
-"""Whole runs of the benchmark configurations against FP64 oracle trajectories.
+"""Whole runs of the benchmark workloads against FP64 oracle trajectories.
 
 Both sides start from the same FP32 state; the oracle then steps in FP64 all
 the way (its state is never rounded to FP32), so the reported error is that of
@@ -17,10 +17,9 @@ some step: the FP64 oracle itself, perturbed each step by random noise of
 error is therefore also reported, and bounded more tightly, over the
 interior rows (INTERIOR rows and more away from either pole).
 
-Config 1: one 256x512 sphere, dt 0.002, all 1000 steps.  Config 2: one
-1024x2048 sphere, dt 0.5/1024, the first 100 of its 500 steps (the FP64
-oracle's 500 steps take minutes on the host).  Config 3: all 64 uneven
-spheres over all 200 steps, stepped together by the engine from 32-step
+One 256x512 sphere, dt 0.002, all 1000 steps.  One 1024x2048 sphere, dt
+0.5/1024, the first 100 of its 500 steps (the FP64 oracle's 500 steps take
+minutes on the host).  The uneven batch: all 64 spheres over all 200 steps, stepped together by the engine from 32-step
 hipGraphs in three concurrent groups, as the benchmark runs them.
 """
 import numpy as np
@@ -55,8 +54,8 @@ def run_errors(nt, got, ref):
     return errs, inner
 
 
-@pytest.mark.parametrize("name,nt,dt,L,steps", [("c1", 256, 0.002, 16, 1000),
-                                                ("c2", 1024, 0.5 / 1024, 32, 100)])
+@pytest.mark.parametrize("name,nt,dt,L,steps", [("s256", 256, 0.002, 16, 1000),
+                                                ("s1024", 1024, 0.5 / 1024, 32, 100)])
 def test_trajectory_single_sphere(name, nt, dt, L, steps):
     state = seeded_state(nt, L=L, seed=1, dens=1)
     log(f"[run] {name} {nt}x{2 * nt} {steps} steps: FP64 oracle trajectory")
@@ -74,7 +73,7 @@ def test_trajectory_single_sphere(name, nt, dt, L, steps):
         assert max(inner) < TOL_INTERIOR
 
 
-def test_trajectory_uneven_batch_config3():
+def test_trajectory_uneven_batch():
     rng = np.random.default_rng(3)
     sizes = [int(n) for n in rng.choice([32, 48, 64, 96, 128, 192, 256], 64)]
     steps = 200
@@ -91,7 +90,7 @@ def test_trajectory_uneven_batch_config3():
         worst[nt] = [max(a, b) for a, b in zip(w, errs + inner)]
     for nt in sorted(worst):
         e = worst[nt]
-        log(f"[run] c3 n_theta={nt} worst after {steps} steps rel L2: u_phi {e[0]:.2e} u_theta {e[1]:.2e} "
+        log(f"[run] uneven n_theta={nt} worst after {steps} steps rel L2: u_phi {e[0]:.2e} u_theta {e[1]:.2e} "
             f"rho {e[2]:.2e} div {e[3]:.2e}; interior u_phi {e[4]:.2e} u_theta {e[5]:.2e} rho {e[6]:.2e}")
     assert max(max(e[:3]) for e in worst.values()) < TOL_RUN
     assert max(max(e[4:]) for e in worst.values()) < TOL_INTERIOR
