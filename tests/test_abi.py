@@ -84,3 +84,34 @@ def test_noalloc_step_rejects_bad_arguments_without_a_gpu():
     bad = K.SimDesc(64, 100, 1.0, 0.01, 1, 0, 0)
     assert L.kam_sim_step_noalloc(C.byref(bad), 1, 1, None, 1, None) == -1
     assert L.kam_last_error().decode()
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of the header's structs have the C compiler's size
+    and field offsets (a field added to a struct on one side only would shift
+    every later field)."""
+    import subprocess
+
+    from kaminogpu_amd import engine as E
+
+    mirrors = {"kam_sim_desc": K.SimDesc, "kam_engine_desc": E.EngineDesc, "kam_req_status": E.ReqStatus,
+               "kam_stats": E.EngineStats}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "kamino.h"', 'int main(void) {']
+    for cname, py in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src, exe = tmp_path / "layout.c", tmp_path / "layout"
+    src.write_text("\n".join(lines) + "\n")
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {}
+    for line in filter(None, out):
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, py in mirrors.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
